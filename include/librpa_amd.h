@@ -1,0 +1,64 @@
+/* Extensions of the MI355X build.  None of these has a counterpart in
+ * LibRPA; they select precision, expose intermediates for verification and
+ * report device timings. */
+#pragma once
+
+#include "librpa_handler.h"
+#include "librpa_options.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIBRPA_AMD_FP64 0
+#define LIBRPA_AMD_FP32 1
+#define LIBRPA_AMD_BF16X3 2
+
+/* Contraction precision of the chi0 build (G, W, T and the final Cs.T
+ * contraction) and of the Ec transforms (chi0 L and L^T X).  The Cholesky
+ * log-determinant, the tau->omega quadrature and all accumulation of Ec stay
+ * in FP64.  Default FP64 / FP64. */
+void librpa_amd_set_precision(LibrpaHandler *h, int chi0_prec, int ec_prec);
+
+/* 0 (default): errors are C++ exceptions thrown through the C ABI as in
+ * LibRPA.  1: errors are caught at the API boundary and recorded; the call
+ * returns NaN / leaves outputs untouched.  Process-wide. */
+void librpa_amd_set_error_mode(int record);
+/* Copy the recorded error message (if any) into buf, clear it, and return 1
+ * if there was one. */
+int librpa_amd_get_last_error(char *buf, int len);
+
+/* Keep chi0(R, tau) and chi0(q, omega) on the device after the Ec call. */
+void librpa_amd_set_keep_chi0(LibrpaHandler *h, int keep);
+/* chi0s block (naux_I x naux_J, row-major) of minimax time point itau for
+ * the BvK cell R; I <= J.  Returns 0 on success. */
+int librpa_amd_get_chi0s_block(LibrpaHandler *h, int itau, int I, int J, const int R[3], double *out);
+/* chi0(q, omega) block, I <= J, row-major re / im.  Returns 0 on success. */
+int librpa_amd_get_chi0q_block(LibrpaHandler *h, int ifreq, int iq, int I, int J, double *re, double *im);
+
+/* Minimax grid of the current mean-field data: tau, tau weights, omega,
+ * omega weights (nfreq each) and the cosine t->f (nfreq x nfreq, omega rows),
+ * cosine f->t (tau rows) and sine t->f matrices.  Returns nfreq. */
+int librpa_amd_get_tfgrids(LibrpaHandler *h, const LibrpaOptions *p_opts, double *tau, double *tau_w,
+                           double *omega, double *omega_w, double *cos_t2f, double *cos_f2t, double *sin_t2f);
+
+/* Device timing table as JSON {"region": [ms, calls, flops, bytes], ...}.
+ * Returns the length written (excluding the terminator). */
+int librpa_amd_profile_json(char *buf, int len);
+void librpa_amd_profile_reset(void);
+
+/* Counters of the last chi0 build: [0] algorithmic chi0 flops (canonical count, one spin,
+ * all tau), [1] executed GEMM flops, [2] chi0 wall seconds, [3] Ec wall seconds,
+ * [4] output blocks (I<=J, R), [5] Cs blocks after folding, [6] n_abf, [7] n_basis.
+ * Returns the number of entries written. */
+int librpa_amd_get_stats(LibrpaHandler *h, double *out, int n);
+
+/* One dense GEMM C = alpha op(A) op(B) + beta C on host arrays through the
+ * device engine (prec as above; tile_cfg 0..6 selects the workgroup tile).
+ * For engine verification.  Returns 0 on success. */
+int librpa_amd_gemm(int prec, int trans_a, int trans_b, int M, int N, int K, double alpha, const double *A, int lda,
+                    const double *B, int ldb, double beta, double *C, int ldc, int tile_cfg);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,76 @@
+// Grouped (persistent-style) GEMM engine for CDNA4 MFMA.
+//
+// Every dense contraction of the chi0 / Ec / Sigma path is expressed as a
+// list of problems C[M x N] = alpha * A[M x K] * B[K x N] + beta * C that
+// share one launch.  Operands live in three base buffers and each problem
+// addresses them through element offsets and leading dimensions, plus two
+// generalisations needed by the atom-pair layouts:
+//   * row indirection: row m of A and C may come from a table of absolute
+//     element offsets (gathered rows, scattered outputs);
+//   * column split on C: output column c goes to (c / ncb) * cbs + c % ncb,
+//     which writes a [j][rows][nu] layout directly from a [rows][(j, nu)] GEMM.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace lra {
+
+enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
+
+inline const char *prec_name(Prec p) {
+    return p == Prec::F64 ? "fp64" : (p == Prec::F32 ? "fp32" : "bf16x3");
+}
+
+struct GemmProb {
+    int M, N, K;
+    int tiles_n;   // filled by GemmBatch::finalize
+    int tile0;     // first global tile of this problem (filled by finalize)
+    int ncb;       // C column-split block (>= N means none)
+    long long cbs; // C column-split stride
+    long long a_off, b_off, c_off;
+    int lda, ldb, ldc;
+    int rt_off;    // offset into the row table, -1 if not row-indirect
+};
+
+// Tile configurations (BM x BN, 256 threads).
+enum TileCfg : int {
+    T64x64 = 0,
+    T128x128 = 1,
+    T32x128 = 2,
+    T128x64 = 3,
+    T64x128 = 4,
+    T16x256 = 5,
+    T32x64 = 6,
+    NUM_TILE_CFG = 7
+};
+int tile_bm(int cfg);
+int tile_bn(int cfg);
+const char *tile_name(int cfg);
+
+// A device-resident list of problems with its row table.
+struct GemmBatch {
+    std::vector<GemmProb> probs;
+    std::vector<long long> rowtab;  // pairs (a_row_off, c_row_off)
+    int cfg = T64x64;
+    bool trans_a = false;  // A stored as A^T (element (m,k) at a_off + k*lda + m)
+    bool trans_b = false;  // B stored as B^T (element (k,n) at b_off + n*ldb + k)
+    bool lower_only = false;
+    int total_tiles = 0;
+    double flops = 0;      // algorithmic flops of one launch (2 M N K summed)
+    DevBuf d_probs, d_rowtab;
+
+    void add(const GemmProb &p) { probs.push_back(p); }
+    void finalize(int cfg_, hipStream_t s = 0);
+    bool empty() const { return probs.empty(); }
+};
+
+GemmProb make_prob(int M, int N, int K, long long a_off, int lda, long long b_off, int ldb,
+                   long long c_off, int ldc);
+
+// Launch all problems of a batch.  A, B, C are the base pointers the offsets refer to.
+void gemm_run(Prec prec, const GemmBatch &b, const void *A, const void *B, void *C, double alpha,
+              double beta, hipStream_t s);
+
+}  // namespace lra

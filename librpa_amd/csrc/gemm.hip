@@ -1,0 +1,341 @@
+// Grouped MFMA GEMM kernels for gfx950.
+//
+// One workgroup (4 waves, 256 threads) computes a BM x BN tile of one
+// problem.  Tiles of all problems of a batch form one flat grid; a binary
+// search over the per-problem tile prefix finds the problem.  The flat tile
+// index is remapped so that each XCD (8 per MI355X, workgroups dispatched
+// round-robin) receives a contiguous range of tiles, which keeps tiles that
+// share A or B panels in the same L2.
+//
+// Operands are staged through LDS as As[m][k] / Bs[n][k] (k contiguous, row
+// pitch padded) with a register prefetch of the next K slice.  Inner products
+// use the 16x16 MFMA families:
+//   F64    v_mfma_f64_16x16x4_f64   (A, B, C in double)
+//   F32    v_mfma_f32_16x16x4_f32   (A, B, C in float)
+//   BF16X3 v_mfma_f32_16x16x32_bf16 three times per K slice with operands split
+//          x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); acc += lo*hi + hi*lo + hi*hi,
+//          operands read as float, accumulation in float.
+#include "gemm.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace lra {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+template <int MODE> struct ModeTraits;
+template <> struct ModeTraits<0> { using T = double; using V = v4d; static constexpr int BK = 16; static constexpr int PAD = 1; };
+template <> struct ModeTraits<1> { using T = float; using V = v4f; static constexpr int BK = 16; static constexpr int PAD = 1; };
+template <> struct ModeTraits<2> { using T = float; using V = v4f; static constexpr int BK = 32; static constexpr int PAD = 4; };
+
+__device__ __forceinline__ unsigned short f2bf_rne(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
+
+__device__ __forceinline__ void split8(const float *x, v8s &hi, v8s &lo) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        unsigned short h = f2bf_rne(x[i]);
+        float r = x[i] - bf2f(h);
+        hi[i] = static_cast<short>(h);
+        lo[i] = static_cast<short>(f2bf_rne(r));
+    }
+}
+
+__device__ __forceinline__ int find_prob(const GemmProb *__restrict__ probs, int nprob, int t) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// XCD-aware remap of the dispatch index: XCD x (= bid % 8) gets a contiguous
+// chunk of logical tiles.
+__device__ __forceinline__ int xcd_remap(int bid, int G) {
+    const int per = G / 8, rem = G % 8;
+    const int x = bid & 7, idx = bid >> 3;
+    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB>
+__global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
+                                                    const typename ModeTraits<MODE>::T *__restrict__ A,
+                                                    const typename ModeTraits<MODE>::T *__restrict__ B,
+                                                    typename ModeTraits<MODE>::T *__restrict__ C,
+                                                    const long long *__restrict__ rowtab,
+                                                    typename ModeTraits<MODE>::T alpha,
+                                                    typename ModeTraits<MODE>::T beta, int lower_only) {
+    using T = typename ModeTraits<MODE>::T;
+    using V = typename ModeTraits<MODE>::V;
+    constexpr int BK = ModeTraits<MODE>::BK;
+    constexpr int PAD = ModeTraits<MODE>::PAD;
+    constexpr int LDK = BK + PAD;
+    constexpr int NT = 256;
+    static_assert(WM * WN == 4, "4 waves");
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    static_assert(FM >= 1 && FN >= 1, "wave tile");
+    constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
+    static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "tile load split");
+
+    __shared__ T As[BM * LDK];
+    __shared__ T Bs[BN * LDK];
+
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= total) return;
+    const int pi = find_prob(probs, nprob, tile);
+    const GemmProb P = probs[pi];
+    const int lt = tile - P.tile0;
+    const int tm = lt / P.tiles_n, tn = lt % P.tiles_n;
+    const int m0 = tm * BM, n0 = tn * BN;
+    if (lower_only && n0 > m0 + BM - 1) return;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const bool rt = P.rt_off >= 0;
+
+    // per-thread A row bases (non-transposed A)
+    long long arow[LA];
+    int am[LA], ak[LA];
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+        const int e = t + i * NT;
+        if (TA) { ak[i] = e / BM; am[i] = e % BM; }
+        else { am[i] = e / BK; ak[i] = e % BK; }
+        const int gm = m0 + am[i];
+        if (!TA) {
+            if (gm < P.M) arow[i] = rt ? rowtab[2ll * (P.rt_off + gm)] : P.a_off + (long long)gm * P.lda;
+            else arow[i] = -1;
+        } else {
+            arow[i] = gm < P.M ? P.a_off + gm : -1;
+        }
+    }
+    int bk[LB], bn[LB];
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+        const int e = t + i * NT;
+        if (TB) { bn[i] = e / BK; bk[i] = e % BK; }
+        else { bk[i] = e / BN; bn[i] = e % BN; }
+    }
+
+    V acc[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
+
+    T ra[LA], rb[LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int gk = k0 + ak[i];
+            T v = 0;
+            if (arow[i] >= 0 && gk < P.K) v = TA ? A[arow[i] + (long long)gk * P.lda] : A[arow[i] + gk];
+            ra[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int gk = k0 + bk[i], gn = n0 + bn[i];
+            T v = 0;
+            if (gk < P.K && gn < P.N)
+                v = TB ? B[P.b_off + (long long)gn * P.ldb + gk] : B[P.b_off + (long long)gk * P.ldb + gn];
+            rb[i] = v;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) As[am[i] * LDK + ak[i]] = ra[i];
+#pragma unroll
+        for (int i = 0; i < LB; ++i) Bs[bn[i] * LDK + bk[i]] = rb[i];
+    };
+
+    const int rowA = wm * WTM + (lane & 15);
+    const int rowB = wn * WTN + (lane & 15);
+    const int kq = lane >> 4;
+
+    gload(0);
+    for (int k0 = 0; k0 < P.K; k0 += BK) {
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (k0 + BK < P.K) gload(k0 + BK);
+        if constexpr (MODE == 0 || MODE == 1) {
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                T fa[FM], fb[FN];
+#pragma unroll
+                for (int a = 0; a < FM; ++a) fa[a] = As[(rowA + a * 16) * LDK + kk + kq];
+#pragma unroll
+                for (int b = 0; b < FN; ++b) fb[b] = Bs[(rowB + b * 16) * LDK + kk + kq];
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        if constexpr (MODE == 0)
+                            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                        else
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 32) {
+                v8s ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+                for (int a = 0; a < FM; ++a) {
+                    const float *p = &As[(rowA + a * 16) * LDK + kk + kq * 8];
+                    float x[8];
+                    const v4f x0 = *reinterpret_cast<const v4f *>(p);
+                    const v4f x1 = *reinterpret_cast<const v4f *>(p + 4);
+                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2]; x[3] = x0[3];
+                    x[4] = x1[0]; x[5] = x1[1]; x[6] = x1[2]; x[7] = x1[3];
+                    split8(x, ah[a], al[a]);
+                }
+#pragma unroll
+                for (int b = 0; b < FN; ++b) {
+                    const float *p = &Bs[(rowB + b * 16) * LDK + kk + kq * 8];
+                    float x[8];
+                    const v4f x0 = *reinterpret_cast<const v4f *>(p);
+                    const v4f x1 = *reinterpret_cast<const v4f *>(p + 4);
+                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2]; x[3] = x0[3];
+                    x[4] = x1[0]; x[5] = x1[1]; x[6] = x1[2]; x[7] = x1[3];
+                    split8(x, bh[b], bl[b]);
+                }
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(v8bf, al[a]), __builtin_bit_cast(v8bf, bh[b]), acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(v8bf, ah[a]), __builtin_bit_cast(v8bf, bl[b]), acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(v8bf, ah[a]), __builtin_bit_cast(v8bf, bh[b]), acc[a][b], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    // epilogue: acc[a][b][r] = C[m0 + wm*WTM + a*16 + 4*(lane/16) + r][n0 + wn*WTN + b*16 + lane%16]
+    const bool split = P.ncb < P.N;
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + wm * WTM + a * 16 + 4 * kq + r;
+            if (gm >= P.M) continue;
+            const long long crow = rt ? rowtab[2ll * (P.rt_off + gm) + 1] : P.c_off + (long long)gm * P.ldc;
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                const int gn = n0 + wn * WTN + b * 16 + (lane & 15);
+                if (gn >= P.N) continue;
+                const long long col = split ? (long long)(gn / P.ncb) * P.cbs + gn % P.ncb : gn;
+                T* dst = C + crow + col;
+                T v = alpha * acc[a][b][r];
+                if (beta != T(0)) v += beta * *dst;
+                *dst = v;
+            }
+        }
+    }
+}
+
+struct TileShape { int bm, bn, wm, wn; const char *name; };
+static const TileShape kTiles[NUM_TILE_CFG] = {
+    {64, 64, 2, 2, "64x64"},   {128, 128, 2, 2, "128x128"}, {32, 128, 1, 4, "32x128"},
+    {128, 64, 2, 2, "128x64"}, {64, 128, 2, 2, "64x128"},   {16, 256, 1, 4, "16x256"},
+    {32, 64, 2, 2, "32x64"},
+};
+int tile_bm(int cfg) { return kTiles[cfg].bm; }
+int tile_bn(int cfg) { return kTiles[cfg].bn; }
+const char *tile_name(int cfg) { return kTiles[cfg].name; }
+
+GemmProb make_prob(int M, int N, int K, long long a_off, int lda, long long b_off, int ldb, long long c_off,
+                   int ldc) {
+    GemmProb p;
+    std::memset(&p, 0, sizeof(p));
+    p.M = M; p.N = N; p.K = K;
+    p.ncb = 1 << 30; p.cbs = 0;
+    p.a_off = a_off; p.lda = lda; p.b_off = b_off; p.ldb = ldb; p.c_off = c_off; p.ldc = ldc;
+    p.rt_off = -1;
+    return p;
+}
+
+void GemmBatch::finalize(int cfg_, hipStream_t s) {
+    cfg = cfg_;
+    const int bm = tile_bm(cfg), bn = tile_bn(cfg);
+    int t0 = 0;
+    flops = 0;
+    for (auto &p : probs) {
+        p.tiles_n = ceil_div(p.N, bn);
+        p.tile0 = t0;
+        t0 += ceil_div(p.M, bm) * p.tiles_n;
+        flops += 2.0 * p.M * p.N * (double)p.K;
+    }
+    total_tiles = t0;
+    d_probs = upload(probs, s);
+    if (!rowtab.empty()) d_rowtab = upload(rowtab, s);
+}
+
+template <int MODE, int CFG, bool TA, bool TB>
+static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
+                       hipStream_t s) {
+    using T = typename ModeTraits<MODE>::T;
+    constexpr int BM = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 32 : (CFG == 3) ? 128 : (CFG == 4) ? 64 : (CFG == 5) ? 16 : 32;
+    constexpr int BN = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 128 : (CFG == 3) ? 64 : (CFG == 4) ? 128 : (CFG == 5) ? 256 : 64;
+    constexpr int WM = (CFG == 2 || CFG == 5) ? 1 : 2;
+    constexpr int WN = 4 / WM;
+    hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB>), dim3(b.total_tiles), dim3(256), 0, s,
+                       b.d_probs.as<GemmProb>(), (int)b.probs.size(), b.total_tiles, static_cast<const T *>(A),
+                       static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(), (T)alpha,
+                       (T)beta, (int)b.lower_only);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int MODE, bool TA, bool TB>
+static void launch_cfg(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
+                       hipStream_t s) {
+    switch (b.cfg) {
+        case 0: launch_one<MODE, 0, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 1: launch_one<MODE, 1, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 2: launch_one<MODE, 2, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 3: launch_one<MODE, 3, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 4: launch_one<MODE, 4, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 5: launch_one<MODE, 5, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 6: launch_one<MODE, 6, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        default: LRA_THROW("bad tile cfg");
+    }
+}
+
+template <int MODE>
+static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
+                        hipStream_t s) {
+    if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);
+    else if (b.trans_a) launch_cfg<MODE, true, false>(b, A, B, C, alpha, beta, s);
+    else if (b.trans_b) launch_cfg<MODE, false, true>(b, A, B, C, alpha, beta, s);
+    else launch_cfg<MODE, false, false>(b, A, B, C, alpha, beta, s);
+}
+
+void gemm_run(Prec prec, const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
+              hipStream_t s) {
+    if (b.total_tiles == 0) return;
+    if (b.trans_a && !b.rowtab.empty()) LRA_THROW("row indirection needs non-transposed A");
+    switch (prec) {
+        case Prec::F64: launch_mode<0>(b, A, B, C, alpha, beta, s); break;
+        case Prec::F32: launch_mode<1>(b, A, B, C, alpha, beta, s); break;
+        case Prec::BF16X3: launch_mode<2>(b, A, B, C, alpha, beta, s); break;
+    }
+}
+
+}  // namespace lra

@@ -1,0 +1,64 @@
+// Host wrappers of the streaming / small kernels of the chi0 -> Ec path.
+#pragma once
+
+#include "common.h"
+
+namespace lra {
+
+// G(R, +-tau) build ----------------------------------------------------------------
+// Zs[(2b + c)][j] = s_b * (c == 0 ? Re U[b][j] : Im U[b][j]) for b in [b0, b1),
+// s_b = exp(min(0, -tau (eps_b - mu))) * w_b with the occupation weights of
+// build_gf_Rt_libri_serial (src/core/chi0.cpp:333-355).  U, eps, wg per k.
+void launch_scale_bands(const double *wre, const double *wim, const double *eps, const double *wg,
+                        double *Zs, int nk, int nbands, int nbasis, int b0, int b1, double tau, double efermi,
+                        int nspin, int nbands_g, hipStream_t s);
+
+// Gpm[R][row][gcol[col]] = sgn * sum_k (cs[R][k] Mre_k[row][col] + sn[R][k] Mim_k[row][col]),
+// Mstack per k laid out [Re rows; Im rows] (2 nbasis x nbasis).  T = double or float.
+template <class T>
+void launch_k2r(const double *Mstack, const double *cs, const double *sn, const int *gcol, T *Gpm, int nk, int nR,
+                int nbasis, double sgn, hipStream_t s);
+
+// Column gather: dst[job][row][c] = src[colsrc[job_cols + c] + row * src_ld] for row < nrows.
+struct GatherJob {
+    long long dst_off;
+    int ncol;
+    int col_off;  // offset into colsrc
+};
+template <class T>
+void launch_gather_cols(const T *src, long long src_ld, T *dst, const GatherJob *jobs, int njobs,
+                        const long long *colsrc, int nrows, int maxcol, hipStream_t s);
+
+// tau -> omega and R -> q of chi0s into dense Hermitian matrices ---------------------
+// For each pair p = (I <= J) with element count na_I * na_J and each omega w of the
+// chunk: val = (2 / nspin) sum_t sum_r gamma[w][t] e^{i 2 pi q.R_r} chi0s[t][eoff[p][r] + e].
+// Output: real (Gamma) n x n, or the real embedding [[Re, -Im], [Im, Re]] (2n x 2n).
+struct PairDesc {
+    int I, J;
+    int naI, naJ;
+    int rowI, colJ;    // ABF offsets
+    long long e0;      // first element in the flat pair-element index
+};
+template <class TS, class TD>
+void launch_chi0_assemble(const TS *chi0s, long long E_tot, int ntau, const PairDesc *pairs, int npairs,
+                          long long n_pair_elems, const long long *eoff, int nR, const double *gam_ph,
+                          int nw, TD *dense, long long dense_stride, int n, bool embed, hipStream_t s);
+
+// Dense helpers --------------------------------------------------------------------
+template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s);
+template <class T> void launch_zero_upper(T *A, int n, int lda, hipStream_t s);
+template <class TS, class TD> void launch_convert(const TS *src, TD *dst, long long n, hipStream_t s);
+void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStream_t s);
+void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s);
+void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s);
+
+// Blocked Cholesky (lower, row-major, FP64): A = L L^T in place.
+// logdet (device scalar) += 2 sum log L_jj; *flag set to 1 on a non-positive pivot.
+// When keep_L is false only the log-determinant is produced.
+struct CholWork {
+    DevBuf linv, panel;
+};
+void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w,
+                     hipStream_t s);
+
+}  // namespace lra

@@ -1,0 +1,411 @@
+// Streaming and small kernels of the chi0 -> Ec path (gfx950).
+#include "kernels.h"
+
+#include <cmath>
+#include <map>
+#include <tuple>
+
+#include "gemm.h"
+
+namespace lra {
+
+// ------------------------------------------------------------------------------------
+// G build helpers
+// ------------------------------------------------------------------------------------
+__global__ void scale_bands_kernel(const double *__restrict__ wre, const double *__restrict__ wim,
+                                   const double *__restrict__ eps, const double *__restrict__ wg,
+                                   double *__restrict__ Zs, int nk, int nbands, int nbasis, int b0, int b1,
+                                   double tau, double efermi, int nspin, int nbands_g) {
+    const long long per_k = (long long)(b1 - b0) * nbasis;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (idx >= per_k || k >= nk) return;
+    const int b = b0 + (int)(idx / nbasis);
+    const int j = (int)(idx % nbasis);
+    const double e = eps[(long long)k * nbands + b];
+    const double g = wg[(long long)k * nbands + b];
+    double w;
+    if (tau > 0) {
+        w = 1.0 / nk - g / 2 * nspin;
+        if (w < 0) w = 0;
+    } else {
+        w = g * 0.5 * nspin;
+    }
+    double sc = -tau * (e - efermi);
+    if (sc > 0) sc = 0;
+    double sb = exp(sc) * w;
+    if (nbands_g >= 0 && b >= nbands_g) sb = 0;
+    const long long src = ((long long)k * nbands + b) * nbasis + j;
+    double *z = Zs + (long long)k * 2 * nbands * nbasis;
+    z[(2ll * b) * nbasis + j] = sb * wre[src];
+    z[(2ll * b + 1) * nbasis + j] = sb * wim[src];
+}
+
+void launch_scale_bands(const double *wre, const double *wim, const double *eps, const double *wg, double *Zs,
+                        int nk, int nbands, int nbasis, int b0, int b1, double tau, double efermi, int nspin,
+                        int nbands_g, hipStream_t s) {
+    if (b1 <= b0) return;
+    const long long per_k = (long long)(b1 - b0) * nbasis;
+    dim3 grid(ceil_div(per_k, 256), nk);
+    hipLaunchKernelGGL(scale_bands_kernel, grid, dim3(256), 0, s, wre, wim, eps, wg, Zs, nk, nbands, nbasis, b0,
+                       b1, tau, efermi, nspin, nbands_g);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+__global__ void k2r_kernel(const double *__restrict__ M, const double *__restrict__ cs, const double *__restrict__ sn,
+                           const int *__restrict__ gcol, T *__restrict__ G, int nk, int nR, int nbasis, double sgn) {
+    const long long nn = (long long)nbasis * nbasis;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nn) return;
+    const int row = (int)(idx / nbasis), col = (int)(idx % nbasis);
+    const long long kstride = 2 * nn;
+    const long long dst = (long long)row * 2 * nbasis + gcol[col];
+    for (int r = 0; r < nR; ++r) {
+        double acc = 0;
+        for (int k = 0; k < nk; ++k) {
+            const double re = M[k * kstride + idx];
+            const double im = M[k * kstride + nn + idx];
+            acc += cs[r * nk + k] * re + sn[r * nk + k] * im;
+        }
+        G[(long long)r * 2 * nn + dst] = static_cast<T>(sgn * acc);
+    }
+}
+
+template <class T>
+void launch_k2r(const double *Mstack, const double *cs, const double *sn, const int *gcol, T *Gpm, int nk, int nR,
+                int nbasis, double sgn, hipStream_t s) {
+    const long long nn = (long long)nbasis * nbasis;
+    hipLaunchKernelGGL((k2r_kernel<T>), dim3(ceil_div(nn, 256)), dim3(256), 0, s, Mstack, cs, sn, gcol, Gpm, nk,
+                       nR, nbasis, sgn);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_k2r<double>(const double *, const double *, const double *, const int *, double *, int, int,
+                                 int, double, hipStream_t);
+template void launch_k2r<float>(const double *, const double *, const double *, const int *, float *, int, int,
+                                int, double, hipStream_t);
+
+template <class T>
+__global__ void gather_cols_kernel(const T *__restrict__ src, long long src_ld, T *__restrict__ dst,
+                                   const GatherJob *__restrict__ jobs, const long long *__restrict__ colsrc,
+                                   int nrows) {
+    const GatherJob J = jobs[blockIdx.y];
+    const long long n = (long long)nrows * J.ncol;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int row = (int)(idx / J.ncol), c = (int)(idx % J.ncol);
+        dst[J.dst_off + idx] = src[colsrc[J.col_off + c] + row * src_ld];
+    }
+}
+
+template <class T>
+void launch_gather_cols(const T *src, long long src_ld, T *dst, const GatherJob *jobs, int njobs,
+                        const long long *colsrc, int nrows, int maxcol, hipStream_t s) {
+    if (njobs == 0) return;
+    const long long n = (long long)nrows * maxcol;
+    const int bx = std::min(ceil_div(n, 256), 2048);
+    hipLaunchKernelGGL((gather_cols_kernel<T>), dim3(bx, njobs), dim3(256), 0, s, src, src_ld, dst, jobs, colsrc,
+                       nrows);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_gather_cols<double>(const double *, long long, double *, const GatherJob *, int,
+                                         const long long *, int, int, hipStream_t);
+template void launch_gather_cols<float>(const float *, long long, float *, const GatherJob *, int,
+                                        const long long *, int, int, hipStream_t);
+
+// ------------------------------------------------------------------------------------
+// chi0 (R, tau) -> dense chi0 (q, omega)
+// ------------------------------------------------------------------------------------
+constexpr int kMaxW = 8;
+
+template <class TS, class TD>
+__global__ void chi0_assemble_kernel(const TS *__restrict__ chi0s, long long E_tot, int ntau,
+                                     const PairDesc *__restrict__ pairs, int npairs, long long nelem,
+                                     const long long *__restrict__ eoff, int nR, const double *__restrict__ gp,
+                                     int nw, TD *__restrict__ dense, long long dstride, int n, int embed) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nelem) return;
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].e0 <= idx) lo = mid; else hi = mid - 1;
+    }
+    const PairDesc P = pairs[lo];
+    const long long e = idx - P.e0;
+    const int mu = (int)(e / P.naJ), nu = (int)(e % P.naJ);
+    double re[kMaxW], im[kMaxW];
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w) { re[w] = 0; im[w] = 0; }
+    for (int r = 0; r < nR; ++r) {
+        const long long off = eoff[(long long)lo * nR + r];
+        if (off < 0) continue;
+        for (int t = 0; t < ntau; ++t) {
+            const double v = static_cast<double>(chi0s[(long long)t * E_tot + off + e]);
+#pragma unroll
+            for (int w = 0; w < kMaxW; ++w) {
+                if (w < nw) {
+                    const double *g = gp + (((long long)w * nR + r) * ntau + t) * 2;
+                    re[w] += g[0] * v;
+                    im[w] += g[1] * v;
+                }
+            }
+        }
+    }
+    const long long row = P.rowI + mu, col = P.colJ + nu;
+    const bool mirror = P.I != P.J;
+#pragma unroll
+    for (int w = 0; w < kMaxW; ++w) {
+        if (w >= nw) break;
+        TD *D = dense + w * dstride;
+        if (!embed) {
+            const long long ld = n;
+            D[row * ld + col] = static_cast<TD>(re[w]);
+            if (mirror) D[col * ld + row] = static_cast<TD>(re[w]);
+        } else {
+            const long long ld = 2ll * n;
+            D[row * ld + col] = static_cast<TD>(re[w]);
+            D[(n + row) * ld + n + col] = static_cast<TD>(re[w]);
+            D[(n + row) * ld + col] = static_cast<TD>(im[w]);
+            D[row * ld + n + col] = static_cast<TD>(-im[w]);
+            if (mirror) {
+                D[col * ld + row] = static_cast<TD>(re[w]);
+                D[(n + col) * ld + n + row] = static_cast<TD>(re[w]);
+                D[(n + col) * ld + row] = static_cast<TD>(-im[w]);
+                D[col * ld + n + row] = static_cast<TD>(im[w]);
+            }
+        }
+    }
+}
+
+template <class TS, class TD>
+void launch_chi0_assemble(const TS *chi0s, long long E_tot, int ntau, const PairDesc *pairs, int npairs,
+                          long long n_pair_elems, const long long *eoff, int nR, const double *gam_ph, int nw,
+                          TD *dense, long long dense_stride, int n, bool embed, hipStream_t s) {
+    if (nw > kMaxW) LRA_THROW("omega chunk too large");
+    hipLaunchKernelGGL((chi0_assemble_kernel<TS, TD>), dim3(ceil_div(n_pair_elems, 256)), dim3(256), 0, s, chi0s,
+                       E_tot, ntau, pairs, npairs, n_pair_elems, eoff, nR, gam_ph, nw, dense, dense_stride, n,
+                       (int)embed);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_chi0_assemble<double, double>(const double *, long long, int, const PairDesc *, int, long long,
+                                                   const long long *, int, const double *, int, double *, long long,
+                                                   int, bool, hipStream_t);
+template void launch_chi0_assemble<float, float>(const float *, long long, int, const PairDesc *, int, long long,
+                                                 const long long *, int, const double *, int, float *, long long,
+                                                 int, bool, hipStream_t);
+template void launch_chi0_assemble<float, double>(const float *, long long, int, const PairDesc *, int, long long,
+                                                  const long long *, int, const double *, int, double *, long long,
+                                                  int, bool, hipStream_t);
+template void launch_chi0_assemble<double, float>(const double *, long long, int, const PairDesc *, int, long long,
+                                                  const long long *, int, const double *, int, float *, long long,
+                                                  int, bool, hipStream_t);
+
+// ------------------------------------------------------------------------------------
+// dense helpers
+// ------------------------------------------------------------------------------------
+template <class T>
+__global__ void set_identity_kernel(T *A, int n, int lda) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    A[(long long)i * lda + j] = (i == j) ? T(1) : T(0);
+}
+template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s) {
+    hipLaunchKernelGGL((set_identity_kernel<T>), dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, n, lda);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_set_identity<double>(double *, int, int, hipStream_t);
+template void launch_set_identity<float>(float *, int, int, hipStream_t);
+
+template <class T>
+__global__ void zero_upper_kernel(T *A, int n, int lda) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    if (j > i) A[(long long)i * lda + j] = T(0);
+}
+template <class T> void launch_zero_upper(T *A, int n, int lda, hipStream_t s) {
+    hipLaunchKernelGGL((zero_upper_kernel<T>), dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, n, lda);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_zero_upper<double>(double *, int, int, hipStream_t);
+template void launch_zero_upper<float>(float *, int, int, hipStream_t);
+
+template <class TS, class TD>
+__global__ void convert_kernel(const TS *__restrict__ src, TD *__restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = static_cast<TD>(src[i]);
+}
+template <class TS, class TD> void launch_convert(const TS *src, TD *dst, long long n, hipStream_t s) {
+    const int bx = std::min(ceil_div(n, 256), 65536);
+    hipLaunchKernelGGL((convert_kernel<TS, TD>), dim3(bx), dim3(256), 0, s, src, dst, n);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_convert<double, float>(const double *, float *, long long, hipStream_t);
+template void launch_convert<float, double>(const float *, double *, long long, hipStream_t);
+template void launch_convert<double, double>(const double *, double *, long long, hipStream_t);
+
+__global__ void copy_lower_f2d_kernel(const float *__restrict__ src, double *__restrict__ dst, int n, int lda) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    if (j <= i) dst[(long long)i * lda + j] = static_cast<double>(src[(long long)i * lda + j]);
+}
+void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStream_t s) {
+    hipLaunchKernelGGL(copy_lower_f2d_kernel, dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, src, dst, n,
+                       lda);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void trace_kernel(const double *__restrict__ A, int n, int lda, double *acc) {
+    __shared__ double red[256];
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += A[(long long)i * lda + i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += red[0];
+}
+void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s) {
+    hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(256), 0, s, A, n, lda, acc);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void copy_block_kernel(const double *__restrict__ src, int lds, double *__restrict__ dst, int ldd,
+                                  int rows, int cols) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)rows * cols) return;
+    const int i = (int)(idx / cols), j = (int)(idx % cols);
+    dst[(long long)i * ldd + j] = src[(long long)i * lds + j];
+}
+void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(copy_block_kernel, dim3(ceil_div((long long)rows * cols, 256)), dim3(256), 0, s, src, lds,
+                       dst, ldd, rows, cols);
+    HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------
+// Blocked Cholesky with log-determinant
+// ------------------------------------------------------------------------------------
+constexpr int CHOL_NB = 64;
+
+// Factor the nb x nb diagonal block at (j0, j0) in LDS, write L11 back, write
+// Linv = L11^{-1} (row-major, CHOL_NB pitch) and accumulate 2 sum log L_jj.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int j0, int nb, double *Linv,
+                                                         double *logdet, int *flag) {
+    __shared__ double L[CHOL_NB][CHOL_NB + 1];
+    __shared__ double X[CHOL_NB][CHOL_NB + 1];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    if (t == 0) bad = 0;
+    for (int e = t; e < nb * nb; e += blockDim.x) {
+        const int i = e / nb, j = e % nb;
+        L[i][j] = A[(long long)(j0 + i) * lda + j0 + j];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (t == 0) {
+            double d = L[j][j];
+            if (!(d > 0.0)) { bad = 1; d = 1.0; }
+            L[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const double djj = L[j][j];
+        for (int i = j + 1 + t; i < nb; i += blockDim.x) L[i][j] /= djj;
+        __syncthreads();
+        const int m = nb - j - 1;
+        for (int e = t; e < m * m; e += blockDim.x) {
+            const int i = j + 1 + e / m, k = j + 1 + e % m;
+            if (k <= i) L[i][k] -= L[i][j] * L[k][j];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < nb * nb; e += blockDim.x) {
+        const int i = e / nb, j = e % nb;
+        if (j <= i) A[(long long)(j0 + i) * lda + j0 + j] = L[i][j];
+    }
+    // inverse of the lower-triangular factor, one column per thread
+    if (t < nb) {
+        const int c = t;
+        for (int r = 0; r < nb; ++r) {
+            if (r < c) { X[r][c] = 0; continue; }
+            double x = (r == c) ? 1.0 : 0.0;
+            for (int k = c; k < r; ++k) x -= L[r][k] * X[k][c];
+            X[r][c] = x / L[r][r];
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < CHOL_NB * CHOL_NB; e += blockDim.x) {
+        const int i = e / CHOL_NB, j = e % CHOL_NB;
+        Linv[e] = (i < nb && j < nb) ? X[i][j] : 0.0;
+    }
+    if (t == 0) {
+        double s = 0;
+        for (int j = 0; j < nb; ++j) s += 2.0 * log(L[j][j]);
+        *logdet += s;
+        if (bad) *flag = 1;
+    }
+}
+
+struct CholPlan {
+    std::vector<int> j0s, nbs;
+    std::vector<GemmBatch> trsm, syrk;
+};
+
+static std::map<std::tuple<int, int, int, hipStream_t>, CholPlan> &chol_plans() {
+    static std::map<std::tuple<int, int, int, hipStream_t>, CholPlan> p;
+    return p;
+}
+
+void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
+    w.linv.reserve(sizeof(double) * CHOL_NB * CHOL_NB);
+    w.panel.reserve(sizeof(double) * (size_t)n * CHOL_NB);
+    auto key = std::make_tuple(n, lda, 0, s);
+    auto &plans = chol_plans();
+    auto it = plans.find(key);
+    if (it == plans.end()) {
+        CholPlan cp;
+        for (int j0 = 0; j0 < n; j0 += CHOL_NB) {
+            const int nb = std::min(CHOL_NB, n - j0);
+            const int m = n - j0 - nb;
+            cp.j0s.push_back(j0);
+            cp.nbs.push_back(nb);
+            GemmBatch tr, sy;
+            if (m > 0) {
+                // P = A21 * Linv^T  (A21: m x nb at (j0+nb, j0))
+                GemmProb p = make_prob(m, nb, nb, (long long)(j0 + nb) * lda + j0, lda, 0, CHOL_NB, 0, nb);
+                tr.trans_b = true;
+                tr.add(p);
+                tr.finalize(T64x64, s);
+                // A22 -= P P^T (lower tiles)
+                GemmProb q = make_prob(m, m, nb, 0, nb, 0, nb, (long long)(j0 + nb) * lda + (j0 + nb), lda);
+                sy.trans_b = true;
+                sy.lower_only = true;
+                sy.add(q);
+                sy.finalize(m >= 2048 ? T128x128 : T64x64, s);
+            }
+            cp.trsm.push_back(std::move(tr));
+            cp.syrk.push_back(std::move(sy));
+        }
+        it = plans.emplace(key, std::move(cp)).first;
+    }
+    CholPlan &cp = it->second;
+    for (size_t b = 0; b < cp.j0s.size(); ++b) {
+        const int j0 = cp.j0s[b], nb = cp.nbs[b];
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, A, lda, j0, nb, w.linv.as<double>(), logdet,
+                           flag);
+        HIP_CHECK(hipGetLastError());
+        const int m = n - j0 - nb;
+        if (m <= 0) break;
+        gemm_run(Prec::F64, cp.trsm[b], A, w.linv.get(), w.panel.get(), 1.0, 0.0, s);
+        if (keep_L) launch_copy_block(w.panel.as<double>(), nb, A + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
+        gemm_run(Prec::F64, cp.syrk[b], w.panel.get(), w.panel.get(), A, -1.0, 1.0, s);
+    }
+    if (keep_L) launch_zero_upper<double>(A, n, lda, s);
+}
+
+}  // namespace lra

@@ -1,0 +1,704 @@
+// Space-time chi0 builder and RPA correlation energy on one GPU.
+//
+// Per imaginary time tau (LibRPA build_chi0_q_space_time_LibRI_routing,
+// src/core/chi0.cpp:1014-1164) the device pipeline is
+//   1. G(R, +-tau): per k, M_k = U^T diag(s) conj(U) as one real GEMM on the
+//      stacked [Re; Im] eigenvectors, then the k -> R sum keeping the real part
+//      (build_gf_Rt_libri_serial, chi0.cpp:303-426).  G(+tau) and G(-tau) are
+//      stored side by side per atom column block: Gpm[R][row][2*ao(J) + s*nb(J) + j].
+//   2. W(K, J, S)[k, (j, nu)] = sum_{(L,R2) in N(J)} G(S+R2)_KL C^J(L,R2): one GEMM
+//      per (J, S, sign) on a gathered G panel, written as W[(s, j)][(K, k)][nu].
+//   3. T(I,J,R)[(K,R1,k,i)][nu] = G_KJ(R-R1) W(I,J,R) + G_IJ(R) W(K,J,R-R1): two row-
+//      indirect GEMM batches with K = 2 nb(J) (both signs at once).
+//   4. chi0s(I,J,R) = Cfin_I * T: one GEMM per output block, K = sum_K nb(I) nb(K).
+// The four terms of LibRI's labels a1b1_a2b2 and a1b2_a2b1 called with
+// (G+, G-) and (G-, G+) (thirdparty/LibRI/include/RI/physics/RPA.hpp:104-138)
+// are the four (sign, term) products of steps 3-4.
+//
+// chi0s(R, tau) of every tau stays resident; the tau -> omega cosine transform
+// and the R -> q sum (chi_libri_ft_ct, chi0.cpp:507-596) are fused into the
+// assembly of dense Hermitian chi0(q, omega) matrices, and Ec
+// (compute_RPA_correlation_blacs_2d{,_gamma_only}, src/core/epsilon.cpp:43-558)
+// uses ln det(1 - V chi0) = ln det(1 - L^T chi0 L) with V = L L^T.
+#include "rpa.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+
+namespace lra {
+
+namespace {
+inline int bvk_mod(int v, int n) { return (((v % n) + 3 * n / 2) % n) - n / 2; }
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+int RpaEngine::cell_index(int x, int y, int z) const {
+    const int x0 = -(period[0] / 2), y0 = -(period[1] / 2), z0 = -(period[2] / 2);
+    return ((x - x0) * period[1] + (y - y0)) * period[2] + (z - z0);
+}
+
+int RpaEngine::fold_index(int x, int y, int z) const {
+    return cell_index(bvk_mod(x, period[0]), bvk_mod(y, period[1]), bvk_mod(z, period[2]));
+}
+
+void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
+    prec = p;
+    ec_prec = pe;
+    ts = (prec == Prec::F64) ? 8 : 4;
+    const auto &mf = ds.mf;
+    natom = (int)ds.nb_wfc.size();
+    if (natom == 0 || ds.nb_aux.size() != (size_t)natom) LRA_THROW("atomic basis sizes not set");
+    nb.assign(ds.nb_wfc.begin(), ds.nb_wfc.end());
+    na.assign(ds.nb_aux.begin(), ds.nb_aux.end());
+    ao.assign(natom + 1, 0);
+    abf.assign(natom + 1, 0);
+    for (int a = 0; a < natom; ++a) { ao[a + 1] = ao[a] + nb[a]; abf[a + 1] = abf[a] + na[a]; }
+    NAO = (int)ao[natom];
+    NABF = (int)abf[natom];
+    if (NAO != mf.nbasis) LRA_THROW("sum of atomic basis sizes != nbasis of the mean-field input");
+    nk = mf.nk;
+    nspin = mf.nspin;
+    nbands = mf.nbands;
+    if (mf.nspinor != 1) LRA_THROW("spinor (SOC) input is outside this build's path");
+    for (int a = 0; a < 3; ++a) period[a] = ds.period[a] > 0 ? ds.period[a] : 1;
+    if (period[0] * period[1] * period[2] != nk) LRA_THROW("k mesh size does not match the number of k points");
+    cells.clear();
+    for (int x = -(period[0] / 2); x <= (period[0] - 1) / 2; ++x)
+        for (int y = -(period[1] / 2); y <= (period[1] - 1) / 2; ++y)
+            for (int z = -(period[2] / 2); z <= (period[2] - 1) / 2; ++z) cells.push_back({x, y, z});
+    nR = (int)cells.size();
+
+    // ---- fold Cs to the BvK cell (RI_Tools::cal_period) ----
+    std::map<std::tuple<int, int, int>, std::vector<double>> folded;  // (I, K, r) -> API layout
+    for (const auto &kv : ds.cs) {
+        const int I = std::get<0>(kv.first), K = std::get<1>(kv.first);
+        const int r = fold_index(std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first));
+        auto &dst = folded[{I, K, r}];
+        if (dst.empty()) dst = kv.second;
+        else for (size_t i = 0; i < dst.size(); ++i) dst[i] += kv.second[i];
+    }
+    n_cs_blocks = (long long)folded.size();
+    nbrs.assign(natom, {});
+    for (const auto &kv : folded) nbrs[std::get<0>(kv.first)].push_back({std::get<1>(kv.first), std::get<2>(kv.first)});
+    koff.assign(natom, {});
+    loff.assign(natom, {});
+    ksum.assign(natom, 0);
+    lsum.assign(natom, 0);
+    cfin_off.assign(natom + 1, 0);
+    cwb_off.assign(natom + 1, 0);
+    for (int I = 0; I < natom; ++I) {
+        long long k = 0, l = 0;
+        for (const auto &n : nbrs[I]) {
+            koff[I].push_back(k);
+            loff[I].push_back(l);
+            k += (long long)nb[n.K] * nb[I];
+            l += nb[n.K];
+        }
+        ksum[I] = k;
+        lsum[I] = l;
+        cfin_off[I + 1] = cfin_off[I] + (long long)na[I] * ksum[I];
+        cwb_off[I + 1] = cwb_off[I] + lsum[I] * nb[I] * na[I];
+    }
+    // packed Cs in the two GEMM layouts
+    {
+        std::vector<double> cfin(cfin_off[natom]), cwb(cwb_off[natom]);
+        for (int I = 0; I < natom; ++I) {
+            for (size_t n = 0; n < nbrs[I].size(); ++n) {
+                const int K = nbrs[I][n].K;
+                const auto &blk = folded.at({I, K, nbrs[I][n].r});
+                const int ni = nb[I], nkk = nb[K], nm = na[I];
+                for (int i = 0; i < ni; ++i)
+                    for (int k = 0; k < nkk; ++k)
+                        for (int mu = 0; mu < nm; ++mu) {
+                            const double v = blk[((size_t)i * nkk + k) * nm + mu];
+                            // Cfin_I[mu][koff + k * nb(I) + i]
+                            cfin[cfin_off[I] + (size_t)mu * ksum[I] + koff[I][n] + (size_t)k * ni + i] = v;
+                            // CWb_I[(loff + k)][i * na(I) + mu]   (I plays the role of J, K of L)
+                            cwb[cwb_off[I] + (size_t)(loff[I][n] + k) * ni * nm + (size_t)i * nm + mu] = v;
+                        }
+            }
+        }
+        if (prec == Prec::F64) {
+            d_Cfin = upload(cfin, stream);
+            d_CWb = upload(cwb, stream);
+        } else {
+            std::vector<float> f1(cfin.begin(), cfin.end()), f2(cwb.begin(), cwb.end());
+            d_Cfin = upload(f1, stream);
+            d_CWb = upload(f2, stream);
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+
+    // ---- outputs (I <= J, all cells), ordered by J ----
+    outs.clear();
+    E_tot = 0;
+    for (int J = 0; J < natom; ++J)
+        for (int I = 0; I <= J; ++I)
+            for (int r = 0; r < nR; ++r) {
+                outs.push_back({I, J, r, E_tot, 0});
+                E_tot += (long long)na[I] * na[J];
+            }
+    std::vector<int> out_index((size_t)natom * natom * nR, -1);
+    for (size_t o = 0; o < outs.size(); ++o)
+        out_index[((size_t)outs[o].I * natom + outs[o].J) * nR + outs[o].r] = (int)o;
+
+    // ---- flop model (canonical count, per spin per tau) ----
+    {
+        double f = 0;
+        for (int J = 0; J < natom; ++J) f += (double)nR * 2.0 * 2.0 * NAO * nb[J] * na[J] * lsum[J];
+        for (const auto &o : outs)
+            for (const auto &n : nbrs[o.I])
+                f += 8.0 * nb[o.I] * nb[o.J] * nb[n.K] * na[o.J] + 2.0 * na[o.I] * nb[o.I] * nb[n.K] * na[o.J];
+        flops_chi0_alg = f * ntau;
+    }
+
+    // ---- chunking over J by memory budget ----
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const double budget = std::min(48e9, 0.35 * (double)free_b);
+    chunks.clear();
+    {
+        int j = 0, o = 0;
+        while (j < natom) {
+            Chunk c;
+            c.j_begin = j;
+            c.o_begin = o;
+            long long g = 0, w = 0, t = 0;
+            int jj = j, oo = o;
+            while (jj < natom) {
+                long long g1 = (long long)nR * 2 * NAO * lsum[jj];
+                long long w1 = (long long)nR * 2 * nb[jj] * NAO * na[jj];
+                long long t1 = 0;
+                int o2 = oo;
+                while (o2 < (int)outs.size() && outs[o2].J == jj) { t1 += ksum[outs[o2].I] * na[jj]; ++o2; }
+                const double bytes = (double)(g + g1 + w + w1 + t + t1) * ts;
+                if (jj > j && bytes > budget) break;
+                g += g1; w += w1; t += t1; oo = o2; ++jj;
+            }
+            c.j_end = jj;
+            c.o_end = oo;
+            c.ghat_elems = g; c.w_elems = w; c.t_elems = t;
+            chunks.push_back(std::move(c));
+            j = jj;
+            o = oo;
+        }
+    }
+
+    const long long gpm_r = (long long)NAO * 2 * NAO;  // elements per cell of Gpm
+    flops_chi0_exec = 0;
+    // reverse neighbour lists: for each K the (I, n) with nbrs[I][n].K == K
+    std::vector<std::vector<std::pair<int, int>>> rev(natom);
+    for (int I = 0; I < natom; ++I)
+        for (size_t n = 0; n < nbrs[I].size(); ++n) rev[nbrs[I][n].K].push_back({I, (int)n});
+    std::set<int> classes(nb.begin(), nb.end());
+
+    for (auto &c : chunks) {
+        // offsets inside the chunk buffers
+        long long go = 0, wo = 0, to = 0;
+        for (int J = c.j_begin; J < c.j_end; ++J)
+            for (int S = 0; S < nR; ++S) {
+                for (int sg = 0; sg < 2; ++sg) { c.ghat_off.push_back(go); go += (long long)NAO * lsum[J]; }
+                c.w_off.push_back(wo);
+                wo += 2LL * nb[J] * NAO * na[J];
+            }
+        for (int o = c.o_begin; o < c.o_end; ++o) { outs[o].toff = to; to += ksum[outs[o].I] * na[outs[o].J]; }
+        auto ghat_at = [&](int J, int S, int sg) { return c.ghat_off[((size_t)(J - c.j_begin) * nR + S) * 2 + sg]; };
+        auto w_at = [&](int J, int S) { return c.w_off[(size_t)(J - c.j_begin) * nR + S]; };
+
+        // gather jobs: Ghat(J,S,sg)[row][(n, l)] = G^sg(S + R2_n)[row][ao(L_n) + l]
+        for (int J = c.j_begin; J < c.j_end; ++J)
+            for (int S = 0; S < nR; ++S)
+                for (int sg = 0; sg < 2; ++sg) {
+                    GatherJob gj;
+                    gj.dst_off = ghat_at(J, S, sg);
+                    gj.ncol = (int)lsum[J];
+                    gj.col_off = (int)c.colsrc.size();
+                    for (const auto &n : nbrs[J]) {
+                        const auto &R2 = cells[n.r];
+                        const auto &Sv = cells[S];
+                        const int rr = fold_index(Sv[0] + R2[0], Sv[1] + R2[1], Sv[2] + R2[2]);
+                        for (int l = 0; l < nb[n.K]; ++l)
+                            c.colsrc.push_back(rr * gpm_r + 2 * ao[n.K] + (long long)sg * nb[n.K] + l);
+                    }
+                    c.max_gcol = std::max(c.max_gcol, gj.ncol);
+                    c.jobs.push_back(gj);
+                }
+        // W build: W[(s', j)][(K,k)][nu], s' = 1 holds W+ (from G+), s' = 0 holds W- (from G-)
+        for (int J = c.j_begin; J < c.j_end; ++J)
+            for (int S = 0; S < nR; ++S)
+                for (int sg = 0; sg < 2; ++sg) {
+                    const int sp = (sg == 0) ? 1 : 0;
+                    GemmProb pr = make_prob(NAO, nb[J] * na[J], (int)lsum[J], ghat_at(J, S, sg), (int)lsum[J],
+                                            cwb_off[J], nb[J] * na[J],
+                                            w_at(J, S) + (long long)sp * nb[J] * NAO * na[J], na[J]);
+                    pr.ncb = na[J];
+                    pr.cbs = (long long)NAO * na[J];
+                    if (pr.K > 0) c.wb.add(pr);
+                }
+        // T_B: per output, rows (n, k) from G_KJ(R - R1), B = W(I, J, R)
+        for (int o = c.o_begin; o < c.o_end; ++o) {
+            const auto &O = outs[o];
+            if (nbrs[O.I].empty()) continue;
+            GemmProb pr = make_prob(0, nb[O.I] * na[O.J], 2 * nb[O.J], 0, 0,
+                                    w_at(O.J, O.r) + ao[O.I] * na[O.J], NAO * na[O.J], 0, 0);
+            pr.rt_off = (int)(c.tb.rowtab.size() / 2);
+            int m = 0;
+            const auto &Rv = cells[O.r];
+            for (size_t n = 0; n < nbrs[O.I].size(); ++n) {
+                const auto &nbk = nbrs[O.I][n];
+                const auto &R1 = cells[nbk.r];
+                const int rr = fold_index(Rv[0] - R1[0], Rv[1] - R1[1], Rv[2] - R1[2]);
+                for (int k = 0; k < nb[nbk.K]; ++k) {
+                    c.tb.rowtab.push_back(rr * gpm_r + (ao[nbk.K] + k) * 2LL * NAO + 2 * ao[O.J]);
+                    c.tb.rowtab.push_back(O.toff + (koff[O.I][n] + (long long)k * nb[O.I]) * na[O.J]);
+                    ++m;
+                }
+            }
+            pr.M = m;
+            c.tb.add(pr);
+        }
+        // T_A: per (J, S, K, class of I), rows (I, n, i) from G_IJ(S + R1), B = W(K, J, S)
+        for (int J = c.j_begin; J < c.j_end; ++J)
+            for (int S = 0; S < nR; ++S)
+                for (int K = 0; K < natom; ++K)
+                    for (int cls : classes) {
+                        GemmProb pr = make_prob(0, nb[K] * na[J], 2 * nb[J], 0, 0, w_at(J, S) + ao[K] * na[J],
+                                                NAO * na[J], 0, 0);
+                        pr.ncb = na[J];
+                        pr.cbs = (long long)cls * na[J];
+                        pr.rt_off = (int)(c.ta.rowtab.size() / 2);
+                        int m = 0;
+                        const auto &Sv = cells[S];
+                        for (const auto &in : rev[K]) {
+                            const int I = in.first, n = in.second;
+                            if (I > J || nb[I] != cls) continue;
+                            const auto &R1 = cells[nbrs[I][n].r];
+                            const int rr = fold_index(Sv[0] + R1[0], Sv[1] + R1[1], Sv[2] + R1[2]);
+                            const int o = out_index[((size_t)I * natom + J) * nR + rr];
+                            for (int i = 0; i < nb[I]; ++i) {
+                                c.ta.rowtab.push_back(rr * gpm_r + (ao[I] + i) * 2LL * NAO + 2 * ao[J]);
+                                c.ta.rowtab.push_back(outs[o].toff + (koff[I][n] + i) * na[J]);
+                                ++m;
+                            }
+                        }
+                        if (m == 0) {
+                            c.ta.rowtab.resize((size_t)pr.rt_off * 2);
+                            continue;
+                        }
+                        pr.M = m;
+                        c.ta.add(pr);
+                    }
+        // final: chi0s(o) = Cfin_I (na_I x ksum_I) * T_o (ksum_I x na_J)
+        for (int o = c.o_begin; o < c.o_end; ++o) {
+            const auto &O = outs[o];
+            if (ksum[O.I] == 0) continue;
+            c.fin.add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff, na[O.J],
+                                O.eoff, na[O.J]));
+        }
+        const bool bf = prec == Prec::BF16X3;
+        (void)bf;
+        c.wb.finalize(T128x128, stream);
+        c.tb.finalize(T128x64, stream);
+        c.ta.finalize(T64x128, stream);
+        c.fin.finalize(T64x64, stream);
+        c.d_jobs = upload(c.jobs, stream);
+        c.d_colsrc = upload(c.colsrc, stream);
+        flops_chi0_exec += c.wb.flops + c.tb.flops + c.ta.flops + c.fin.flops;
+    }
+    flops_chi0_exec *= ntau;
+    size_t max_g = 0, max_w = 0, max_t = 0;
+    for (auto &c : chunks) {
+        max_g = std::max(max_g, (size_t)c.ghat_elems);
+        max_w = std::max(max_w, (size_t)c.w_elems);
+        max_t = std::max(max_t, (size_t)c.t_elems);
+    }
+    d_Ghat.alloc(max_g * ts);
+    d_W.alloc(max_w * ts);
+    d_T.alloc(max_t * ts);
+
+    // ---- G build data ----
+    {
+        std::vector<double> zz((size_t)nk * 2 * nbands * 2 * NAO), wre((size_t)nk * nbands * NAO),
+            wim((size_t)nk * nbands * NAO);
+        d_eps.alloc(sizeof(double) * (size_t)nspin * nk * nbands);
+        d_wg.alloc(sizeof(double) * (size_t)nspin * nk * nbands);
+        HIP_CHECK(hipMemcpyAsync(d_eps.get(), mf.eps.data(), d_eps.bytes(), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_wg.get(), mf.wg.data(), d_wg.bytes(), hipMemcpyHostToDevice, stream));
+        d_ZZ.alloc(sizeof(double) * zz.size() * nspin);
+        d_wre.alloc(sizeof(double) * wre.size() * nspin);
+        d_wim.alloc(sizeof(double) * wim.size() * nspin);
+        for (int s = 0; s < nspin; ++s) {
+            for (int k = 0; k < nk; ++k) {
+                const auto &U = mf.wfc[(size_t)s * nk + k];
+                if (U.size() != (size_t)nbands * NAO) LRA_THROW("eigenvectors missing for a k point");
+                double *z = &zz[(size_t)k * 2 * nbands * 2 * NAO];
+                for (int b = 0; b < nbands; ++b)
+                    for (int j = 0; j < NAO; ++j) {
+                        const double x = U[(size_t)b * NAO + j].real(), y = U[(size_t)b * NAO + j].imag();
+                        z[(size_t)(2 * b) * 2 * NAO + j] = x;
+                        z[(size_t)(2 * b) * 2 * NAO + NAO + j] = y;
+                        z[(size_t)(2 * b + 1) * 2 * NAO + j] = y;
+                        z[(size_t)(2 * b + 1) * 2 * NAO + NAO + j] = -x;
+                        wre[((size_t)k * nbands + b) * NAO + j] = x;
+                        wim[((size_t)k * nbands + b) * NAO + j] = y;
+                    }
+            }
+            HIP_CHECK(hipMemcpy(d_ZZ.as<double>() + zz.size() * s, zz.data(), zz.size() * 8, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(d_wre.as<double>() + wre.size() * s, wre.data(), wre.size() * 8, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(d_wim.as<double>() + wim.size() * s, wim.data(), wim.size() * 8, hipMemcpyHostToDevice));
+        }
+        d_Zs.alloc(sizeof(double) * (size_t)nk * 2 * nbands * NAO);
+        d_Mst.alloc(sizeof(double) * (size_t)nk * 2 * NAO * NAO);
+        d_Gpm.alloc(ts * (size_t)nR * NAO * 2 * NAO);
+        std::vector<double> cs((size_t)nR * nk), sn((size_t)nR * nk);
+        for (int r = 0; r < nR; ++r)
+            for (int k = 0; k < nk; ++k) {
+                const double *kf = &ds.kfrac[(size_t)k * 3];
+                const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+                cs[(size_t)r * nk + k] = std::cos(ang);
+                sn[(size_t)r * nk + k] = std::sin(ang);
+            }
+        d_cs = upload(cs, stream);
+        d_sn = upload(sn, stream);
+        std::vector<int> gcol(2 * (size_t)NAO);
+        for (int J = 0; J < natom; ++J)
+            for (int j = 0; j < nb[J]; ++j)
+                for (int sg = 0; sg < 2; ++sg) gcol[(size_t)sg * NAO + ao[J] + j] = (int)(2 * ao[J] + sg * nb[J] + j);
+        d_gcol = upload(gcol, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // ---- chi0s store and transform descriptors ----
+    d_chi0s.alloc(ts * (size_t)E_tot * ntau);
+    pairs.clear();
+    pair_eoff.clear();
+    n_pair_elems = 0;
+    for (int I = 0; I < natom; ++I)
+        for (int J = I; J < natom; ++J) {
+            PairDesc pd;
+            pd.I = I; pd.J = J; pd.naI = na[I]; pd.naJ = na[J];
+            pd.rowI = (int)abf[I]; pd.colJ = (int)abf[J];
+            pd.e0 = n_pair_elems;
+            n_pair_elems += (long long)na[I] * na[J];
+            pairs.push_back(pd);
+            for (int r = 0; r < nR; ++r) {
+                const int o = out_index[((size_t)I * natom + J) * nR + r];
+                pair_eoff.push_back(o >= 0 ? outs[o].eoff : -1);
+            }
+        }
+    d_pairs = upload(pairs, stream);
+    d_pair_eoff = upload(pair_eoff, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
+    const auto &mf = ds.mf;
+    auto &prof = profiler();
+    const double t0 = now_s();
+    const int ntau = g.n;
+    // band ranges with non-zero weight for each sign
+    for (int sg = 0; sg < 2; ++sg) {
+        int lo = nbands, hi = 0;
+        for (int s = 0; s < nspin; ++s)
+            for (int k = 0; k < nk; ++k)
+                for (int b = 0; b < nbands; ++b) {
+                    const double wgv = mf.wg[((size_t)s * nk + k) * nbands + b];
+                    double w = (sg == 0) ? 1.0 / nk - wgv / 2 * nspin : wgv * 0.5 * nspin;
+                    if (w < 0) w = 0;
+                    if (nbands_g >= 0 && b >= nbands_g) w = 0;
+                    if (w > 0) { lo = std::min(lo, b); hi = std::max(hi, b + 1); }
+                }
+        if (hi <= lo) { lo = 0; hi = 0; }
+        band_lo[sg] = lo;
+        band_hi[sg] = hi;
+        GemmBatch &b = gb[sg];
+        b = GemmBatch();
+        b.trans_a = true;
+        if (hi > lo)
+            for (int k = 0; k < nk; ++k)
+                b.add(make_prob(2 * NAO, NAO, 2 * (hi - lo), (long long)k * 2 * nbands * 2 * NAO + 2LL * lo * 2 * NAO,
+                                2 * NAO, (long long)k * 2 * nbands * NAO + 2LL * lo * NAO, NAO,
+                                (long long)k * 2 * NAO * NAO, NAO));
+        b.finalize(T128x128, stream);
+    }
+    const long long gpm_bytes = (long long)nR * NAO * 2 * NAO * ts;
+    HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
+    for (int it = 0; it < ntau; ++it) {
+        const double tau = g.tau[it];
+        for (int s = 0; s < nspin; ++s) {
+            const size_t zz_s = (size_t)nk * 2 * nbands * 2 * NAO * s;
+            const size_t w_s = (size_t)nk * nbands * NAO * s;
+            const size_t e_s = (size_t)nk * nbands * s;
+            prof.begin("gf_build", stream);
+            for (int sg = 0; sg < 2; ++sg) {
+                const double t = sg == 0 ? tau : -tau;
+                if (band_hi[sg] > band_lo[sg]) {
+                    launch_scale_bands(d_wre.as<double>() + w_s, d_wim.as<double>() + w_s, d_eps.as<double>() + e_s,
+                                       d_wg.as<double>() + e_s, d_Zs.as<double>(), nk, nbands, NAO, band_lo[sg],
+                                       band_hi[sg], t, mf.efermi, nspin, nbands_g, stream);
+                    gemm_run(Prec::F64, gb[sg], d_ZZ.as<double>() + zz_s, d_Zs.get(), d_Mst.get(), 1.0, 0.0, stream);
+                } else {
+                    HIP_CHECK(hipMemsetAsync(d_Mst.get(), 0, d_Mst.bytes(), stream));
+                }
+                const double sgn = t < 0 ? -1.0 : 1.0;
+                if (prec == Prec::F64)
+                    launch_k2r<double>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
+                                       d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<double>(), nk, nR, NAO, sgn,
+                                       stream);
+                else
+                    launch_k2r<float>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
+                                      d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<float>(), nk, nR, NAO, sgn,
+                                      stream);
+            }
+            prof.end("gf_build", stream, gb[0].flops + gb[1].flops, (double)gpm_bytes);
+            char *chi_base = d_chi0s.as<char>() + (size_t)it * E_tot * ts;
+            for (auto &c : chunks) {
+                prof.begin("chi0_gather", stream);
+                if (prec == Prec::F64)
+                    launch_gather_cols<double>(d_Gpm.as<double>(), 2LL * NAO, d_Ghat.as<double>(), c.d_jobs.as<GatherJob>(),
+                                               (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
+                else
+                    launch_gather_cols<float>(d_Gpm.as<float>(), 2LL * NAO, d_Ghat.as<float>(), c.d_jobs.as<GatherJob>(),
+                                              (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
+                prof.end("chi0_gather", stream, 0, 2.0 * c.ghat_elems * ts);
+                prof.begin("chi0_W", stream);
+                gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
+                prof.end("chi0_W", stream, c.wb.flops);
+                prof.begin("chi0_T", stream);
+                gemm_run(prec, c.tb, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 1.0, stream);
+                prof.end("chi0_T", stream, c.tb.flops + c.ta.flops);
+                prof.begin("chi0_contract", stream);
+                gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                prof.end("chi0_contract", stream, c.fin.flops);
+            }
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    ntau_done = ntau;
+    t_chi0 = now_s() - t0;
+}
+
+namespace {
+// IBZ k list as in PeriodicBoundaryData::set_ibz_mapping (src/core/pbc.cpp:110-153)
+void ibz_list(const Dataset &ds, int nk, std::vector<int> &ibz_k, std::vector<double> &w) {
+    std::vector<int> map = ds.ibz_map;
+    if ((int)map.size() != nk) { map.resize(nk); for (int k = 0; k < nk; ++k) map[k] = k; }
+    std::map<int, int> pos;
+    ibz_k.clear();
+    w.clear();
+    for (int k = 0; k < nk; ++k) {
+        const int kb = map[k];
+        auto it = pos.find(kb);
+        if (it == pos.end()) { pos[kb] = (int)ibz_k.size(); ibz_k.push_back(kb); w.push_back(1.0 / nk); }
+        else w[it->second] += 1.0 / nk;
+    }
+}
+}  // namespace
+
+RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
+    auto &prof = profiler();
+    const double t0 = now_s();
+    RpaResult res;
+    const bool embed = nk > 1;
+    const int n = NABF, n2 = embed ? 2 * NABF : NABF;
+    const size_t nn = (size_t)n2 * n2;
+    std::vector<int> ibz;
+    std::vector<double> wq;
+    ibz_list(ds, nk, ibz, wq);
+    res.qcontrib.assign(ibz.size(), cplx(0, 0));
+    const bool f64 = ec_prec == Prec::F64;
+    const size_t es = f64 ? 8 : 4;
+    const int WCH = 8;
+    DevBuf dV(nn * 8), dL32, dD(nn * es * WCH), dX(nn * es), dM(nn * es), dM64, dacc;
+    if (!f64) { dL32.alloc(nn * 4); dM64.alloc(nn * 8); }
+    const int nw = g.n;
+    const int nq = (int)ibz.size();
+    dacc.alloc(sizeof(double) * 2 * nq * nw + 64);
+    HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
+    int *dflag = reinterpret_cast<int *>(dacc.as<char>() + sizeof(double) * 2 * nq * nw);
+    CholWork cw;
+    // GEMM batches: X = D L ; M = I - L^T X (lower tiles)
+    GemmBatch bx, bm;
+    bx.add(make_prob(n2, n2, n2, 0, n2, 0, n2, 0, n2));
+    bx.finalize(n2 >= 1024 ? T128x128 : T64x64, stream);
+    bm.trans_a = true;
+    bm.lower_only = true;
+    bm.add(make_prob(n2, n2, n2, 0, n2, 0, n2, 0, n2));
+    bm.finalize(n2 >= 1024 ? T128x128 : T64x64, stream);
+    std::vector<double> gp;
+    DevBuf dgp;
+    std::vector<double> Vh(nn);
+    for (int iq = 0; iq < nq; ++iq) {
+        const int kq = ibz[iq];
+        // dense V(q) from upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
+        std::fill(Vh.begin(), Vh.end(), 0.0);
+        for (const auto &kv : ds.vq) {
+            const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
+            if (ik != kq || I > J) continue;
+            const auto &blk = kv.second;
+            for (int mu = 0; mu < na[I]; ++mu)
+                for (int nu = 0; nu < na[J]; ++nu) {
+                    const cplx v = blk[(size_t)mu * na[J] + nu];
+                    const size_t r = abf[I] + mu, c = abf[J] + nu;
+                    if (!embed) {
+                        Vh[r * n2 + c] = v.real();
+                        if (I != J) Vh[c * n2 + r] = v.real();
+                    } else {
+                        Vh[r * n2 + c] = v.real();
+                        Vh[(n + r) * n2 + n + c] = v.real();
+                        Vh[(n + r) * n2 + c] = v.imag();
+                        Vh[r * n2 + n + c] = -v.imag();
+                        if (I != J) {
+                            Vh[c * n2 + r] = v.real();
+                            Vh[(n + c) * n2 + n + r] = v.real();
+                            Vh[(n + c) * n2 + r] = -v.imag();
+                            Vh[c * n2 + n + r] = v.imag();
+                        }
+                    }
+                }
+        }
+        HIP_CHECK(hipMemcpyAsync(dV.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, stream));
+        prof.begin("ec_chol_v", stream);
+        cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream);
+        prof.end("ec_chol_v", stream, (double)n2 * n2 * n2 / 3.0);
+        const void *Lp = dV.get();
+        if (!f64) {
+            launch_convert<double, float>(dV.as<double>(), dL32.as<float>(), (long long)nn, stream);
+            Lp = dL32.get();
+        }
+        const double *kf = &ds.kfrac[(size_t)kq * 3];
+        for (int w0 = 0; w0 < nw; w0 += WCH) {
+            const int cw_n = std::min(WCH, nw - w0);
+            gp.assign((size_t)cw_n * nR * ntau_done * 2, 0.0);
+            for (int w = 0; w < cw_n; ++w)
+                for (int r = 0; r < nR; ++r) {
+                    const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+                    for (int t = 0; t < ntau_done; ++t) {
+                        const double gam = g.cos_t2f[(size_t)(w0 + w) * g.n + t] * 2.0 / nspin;
+                        gp[(((size_t)w * nR + r) * ntau_done + t) * 2] = gam * std::cos(ang);
+                        gp[(((size_t)w * nR + r) * ntau_done + t) * 2 + 1] = gam * std::sin(ang);
+                    }
+                }
+            dgp = upload(gp, stream);
+            prof.begin("chi0_transform", stream);
+            const double tbytes = (double)ts * E_tot * ntau_done + (double)es * cw_n * (double)E_tot / nR * (embed ? 4 : 1) * 2;
+            if (prec == Prec::F64 && f64)
+                launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                     (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                     dgp.as<double>(), cw_n, dD.as<double>(), (long long)nn, n, embed, stream);
+            else if (prec == Prec::F64)
+                launch_chi0_assemble<double, float>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                    (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                    dgp.as<double>(), cw_n, dD.as<float>(), (long long)nn, n, embed, stream);
+            else if (f64)
+                launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                    (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                    dgp.as<double>(), cw_n, dD.as<double>(), (long long)nn, n, embed, stream);
+            else
+                launch_chi0_assemble<float, float>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                   (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                   dgp.as<double>(), cw_n, dD.as<float>(), (long long)nn, n, embed, stream);
+            prof.end("chi0_transform", stream, 0, tbytes);
+            for (int w = 0; w < cw_n; ++w) {
+                const int iw = w0 + w;
+                char *Dw = dD.as<char>() + (size_t)w * nn * es;
+                double *acc = dacc.as<double>() + 2 * ((size_t)iq * nw + iw);
+                prof.begin("ec_gemm", stream);
+                gemm_run(ec_prec, bx, Dw, Lp, dX.get(), 1.0, 0.0, stream);
+                if (f64) launch_set_identity<double>(dM.as<double>(), n2, n2, stream);
+                else launch_set_identity<float>(dM.as<float>(), n2, n2, stream);
+                gemm_run(ec_prec, bm, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
+                prof.end("ec_gemm", stream, bx.flops + bm.flops * 0.5);
+                double *M64 = dM.as<double>();
+                if (!f64) {
+                    launch_copy_lower_f2d(dM.as<float>(), dM64.as<double>(), n2, n2, stream);
+                    M64 = dM64.as<double>();
+                }
+                prof.begin("ec_logdet", stream);
+                launch_trace_acc(M64, n2, n2, acc, stream);
+                cholesky_logdet(M64, n2, n2, acc + 1, dflag, false, cw, stream);
+                prof.end("ec_logdet", stream, (double)n2 * n2 * n2 / 3.0);
+            }
+        }
+    }
+    std::vector<double> hacc(2 * (size_t)nq * nw);
+    int hflag[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(hacc.data(), dacc.get(), hacc.size() * 8, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(hflag, dflag, sizeof(hflag), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (hflag[1]) LRA_THROW("Coulomb matrix is not positive definite (Cholesky pivot <= 0)");
+    if (hflag[0]) LRA_THROW("1 - V chi0 is not positive definite at some (q, omega): chi0 is not negative semi-definite");
+    const double half = embed ? 0.5 : 1.0;
+    for (int iq = 0; iq < nq; ++iq)
+        for (int iw = 0; iw < nw; ++iw) {
+            const double tr = hacc[2 * ((size_t)iq * nw + iw)];
+            const double ld = hacc[2 * ((size_t)iq * nw + iw) + 1];
+            const double trpi = (n2 - tr) * half;
+            const double val = trpi + ld * half;
+            res.qcontrib[iq] += cplx(val * g.omega_w[iw] * wq[iq] / (2 * M_PI), 0.0);
+        }
+    for (auto &c : res.qcontrib) res.ec_re += c.real();
+    t_ec = now_s() - t0;
+    return res;
+}
+
+void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {
+    for (const auto &o : outs)
+        if (o.I == I && o.J == J && o.r == r) {
+            const size_t n = (size_t)na[I] * na[J];
+            const size_t off = (size_t)it * E_tot + o.eoff;
+            if (prec == Prec::F64) {
+                HIP_CHECK(hipMemcpy(out, d_chi0s.as<double>() + off, n * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> tmp(n);
+                HIP_CHECK(hipMemcpy(tmp.data(), d_chi0s.as<float>() + off, n * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
+            }
+            return;
+        }
+    LRA_THROW("chi0s block not found");
+}
+
+void RpaEngine::chi0q_block(const Dataset &ds, const TFGrid &g, int iw, int iq, int I, int J, double *re,
+                            double *im) {
+    const bool embed = nk > 1;
+    const int n = NABF, n2 = embed ? 2 * NABF : NABF;
+    DevBuf dD((size_t)n2 * n2 * 8);
+    std::vector<double> gp((size_t)nR * ntau_done * 2);
+    const double *kf = &ds.kfrac[(size_t)iq * 3];
+    for (int r = 0; r < nR; ++r) {
+        const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+        for (int t = 0; t < ntau_done; ++t) {
+            const double gam = g.cos_t2f[(size_t)iw * g.n + t] * 2.0 / nspin;
+            gp[((size_t)r * ntau_done + t) * 2] = gam * std::cos(ang);
+            gp[((size_t)r * ntau_done + t) * 2 + 1] = gam * std::sin(ang);
+        }
+    }
+    DevBuf dgp = upload(gp, stream);
+    if (prec == Prec::F64)
+        launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                             (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                             dgp.as<double>(), 1, dD.as<double>(), 0, n, embed, stream);
+    else
+        launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                            (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                            dgp.as<double>(), 1, dD.as<double>(), 0, n, embed, stream);
+    std::vector<double> h((size_t)n2 * n2);
+    HIP_CHECK(hipMemcpyAsync(h.data(), dD.get(), h.size() * 8, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (int mu = 0; mu < na[I]; ++mu)
+        for (int nu = 0; nu < na[J]; ++nu) {
+            const size_t r = abf[I] + mu, c = abf[J] + nu;
+            re[(size_t)mu * na[J] + nu] = h[r * n2 + c];
+            im[(size_t)mu * na[J] + nu] = embed ? h[(n + r) * n2 + c] : 0.0;
+        }
+}
+
+}  // namespace lra

@@ -1,0 +1,83 @@
+// Space-time chi0 builder and RPA correlation energy on one GPU.
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "dataset.h"
+#include "gemm.h"
+#include "kernels.h"
+
+namespace lra {
+
+struct RpaResult {
+    double ec_re = 0;
+    std::vector<cplx> qcontrib;  // per IBZ k
+};
+
+struct RpaEngine {
+    // ---- static plan (tau independent) ----
+    Prec prec = Prec::F64, ec_prec = Prec::F64;
+    int natom = 0, NAO = 0, NABF = 0, nR = 0, nk = 0, nspin = 1, nbands = 0;
+    int period[3] = {1, 1, 1};
+    std::vector<int> nb, na;
+    std::vector<long long> ao, abf;
+    std::vector<std::array<int, 3>> cells;
+
+    struct Nbr { int K, r; };
+    std::vector<std::vector<Nbr>> nbrs;           // folded Cs neighbours of each atom
+    std::vector<std::vector<long long>> koff;     // row offset (in T / Cfin columns) per neighbour
+    std::vector<std::vector<long long>> loff;     // row offset in CWb per neighbour
+    std::vector<long long> ksum, lsum, cfin_off, cwb_off;
+    long long n_cs_blocks = 0;
+
+    struct Out { int I, J, r; long long eoff, toff; };
+    std::vector<Out> outs;
+    long long E_tot = 0;
+
+    struct Chunk {
+        int j_begin, j_end;          // atoms J in [j_begin, j_end)
+        int o_begin, o_end;          // outputs
+        long long ghat_elems = 0, w_elems = 0, t_elems = 0;
+        std::vector<long long> ghat_off, w_off;  // per (J - j_begin, S, sigma) / (J - j_begin, S)
+        std::vector<GatherJob> jobs;
+        std::vector<long long> colsrc;
+        DevBuf d_jobs, d_colsrc;
+        int max_gcol = 0;
+        GemmBatch wb, tb, ta, fin;
+    };
+    std::vector<Chunk> chunks;
+
+    // G build
+    int band_lo[2] = {0, 0}, band_hi[2] = {0, 0};  // per sign (0: +tau, 1: -tau)
+    GemmBatch gb[2];
+
+    // ---- device data ----
+    size_t ts = 8;  // storage element size (8 fp64, 4 fp32)
+    DevBuf d_Cfin, d_CWb, d_ZZ, d_Zs, d_Mst, d_Gpm, d_Ghat, d_W, d_T, d_chi0s;
+    DevBuf d_wre, d_wim, d_eps, d_wg, d_cs, d_sn, d_gcol;
+    hipStream_t stream = 0;
+
+    // transform / Ec
+    std::vector<PairDesc> pairs;
+    std::vector<long long> pair_eoff;  // [pair][r]
+    long long n_pair_elems = 0;
+    DevBuf d_pairs, d_pair_eoff;
+
+    // counters
+    double flops_chi0_alg = 0;   // canonical count per spin, all tau
+    double flops_chi0_exec = 0;
+    double t_chi0 = 0, t_ec = 0;
+    int ntau_done = 0;
+
+    void plan(const Dataset &ds, Prec p, Prec pe, int ntau);
+    void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);
+    RpaResult ec(const Dataset &ds, const TFGrid &g);
+    // dense chi0(q, omega) block for verification
+    void chi0q_block(const Dataset &ds, const TFGrid &g, int iw, int iq, int I, int J, double *re, double *im);
+    void chi0s_block(int it, int I, int J, int r, double *out);
+    int cell_index(int x, int y, int z) const;
+    int fold_index(int x, int y, int z) const;
+};
+
+}  // namespace lra

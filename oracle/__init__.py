@@ -1,0 +1,12 @@
+"""CPU FP64 oracle for the space-time chi0 -> RPA Ec and G0W0 Sigma path.
+
+TEST INFRASTRUCTURE ONLY.  This package restates LibRPA's algorithm in numpy
+so that the HIP implementation in ``librpa_amd`` can be checked against it.
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it, and only as the checker / the timed CPU baseline;
+nothing in the shipped library calls into it.
+
+Every function cites the reference file:line it follows.  Pins (what fixes
+each function independently of this restatement) are listed in the module
+docstrings and exercised by ``tests/test_oracle_*.py``.
+"""

@@ -1,0 +1,108 @@
+"""Pins of the FP64 oracle (CPU only).
+
+* factorised chi0 == brute-force LibRI label sums (LRI-loop3-test.hpp:353-393) on
+  mixed-species k-point systems, including a planted index mistake that must fail;
+* G(R, 0+) - G(R, 0-) = delta_{R,0} I (completeness of the unitary eigenvectors),
+  and Tr G(0, 0-) = -n_occ: fixes occupation weights, sign and phase convention;
+* ln det(1 - Pi) via LU == sum ln(1 - lambda_i) over the eigenvalues of Pi;
+* Ec <= 0 for the 2-atom workload and the per-k contributions sum to Ec.
+"""
+import numpy as np
+import pytest
+
+from librpa_amd import synth
+from oracle import rpa, tfgrids
+
+
+def _sys(**kw):
+    return synth.tiny(**kw)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0),
+    dict(natom=3, period=(1, 2, 2), nao=(2, 3), nabf=(3, 4), nocc=(1, 1), a=6.5, seed=11),
+    dict(natom=2, period=(1, 1, 1), nao=(3, 3), nabf=(4, 4), nocc=(1, 2), a=9.0, seed=5),
+])
+def test_factorized_equals_bruteforce(kw):
+    s = _sys(**kw)
+    cs = rpa.fold_cs(s)
+    Rs = rpa.r_grid(s.period)
+    pairs = rpa.upper_pairs(s.natom)
+    Gp = rpa.gf_R(s, 0.37, Rs)
+    Gn = rpa.gf_R(s, -0.37, Rs)
+    a = rpa.chi0s_bruteforce(s, cs, Gp, Gn, pairs, Rs)
+    b = rpa.chi0s_factorized(s, cs, Gp, Gn, pairs, Rs)
+    for k in a:
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * max(1.0, np.abs(a[k]).max()), k
+
+
+def test_bruteforce_detects_planted_shift():
+    # a plausible indexing mistake (G at -R instead of R) must change chi0 (needs a period >= 3)
+    s = _sys(natom=2, period=(3, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0)
+    cs = rpa.fold_cs(s)
+    Rs = rpa.r_grid(s.period)
+    Gp = rpa.gf_R(s, 0.37, Rs)
+    Gn = rpa.gf_R(s, -0.37, Rs)
+    good = rpa.chi0s_bruteforce(s, cs, Gp, Gn, rpa.upper_pairs(2), Rs)
+    orig = rpa._gblk
+    try:
+        def bad(G, ao, I, J, R, period):
+            return orig(G, ao, I, J, -np.asarray(R), period)
+        rpa._gblk = bad
+        wrong = rpa.chi0s_bruteforce(s, cs, Gp, Gn, rpa.upper_pairs(2), Rs)
+    finally:
+        rpa._gblk = orig
+    diff = max(np.abs(good[k] - wrong[k]).max() for k in good)
+    assert diff > 1e-6
+
+
+def test_gf_anticommutator_and_occupation():
+    s = synth.cell2(with_vq=False)
+    Rs = rpa.r_grid(s.period)
+    eps = 1e-12
+    Gp = rpa.gf_R(s, eps, Rs)
+    Gn = rpa.gf_R(s, -eps, Rs)
+    for R in Rs:
+        d = Gp[R] - Gn[R]
+        ref = np.eye(s.nbasis) if R == (0, 0, 0) else np.zeros_like(d)
+        assert np.abs(d - ref).max() < 1e-10
+    nocc = int(round(s.occ[0, 0].sum() / 2))
+    assert abs(np.trace(Gn[(0, 0, 0)]) + nocc) < 1e-10
+    assert abs(np.trace(Gp[(0, 0, 0)]) - (s.nbands - nocc)) < 1e-10
+
+
+def test_lndet_equals_eigen_sum():
+    rng = np.random.default_rng(3)
+    n = 40
+    b = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    V = b @ b.conj().T / n + 1e-3 * np.eye(n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    chi = -(x @ x.conj().T) / n
+    Pi = V @ chi
+    lam = np.linalg.eigvals(Pi)
+    ref = np.sum(np.log(1 - lam)).real
+    assert abs(rpa.ln_det_ref(np.eye(n) - Pi).real - ref) < 1e-9 * abs(ref)
+    Pr = V.real @ chi.real
+    lam = np.linalg.eigvals(Pr)
+    assert abs(rpa.ln_det_ref(np.eye(n) - Pr) - np.sum(np.log(1 - lam)).real) < 1e-9
+
+
+def test_transform_matches_explicit_sum():
+    s = _sys(natom=2, period=(2, 1, 1), nao=(2, 2), nabf=(3, 3), nocc=(1, 1), a=7.0, nfreq=6)
+    emin, emax = tfgrids.e_min_max(s.eps, s.occ, 1, s.nk)
+    g = tfgrids.minimax(6, emin, emax)
+    chi0s = rpa.chi0s_all_tau(s, g)
+    q = rpa.chi0_q_omega(s, g, chi0s)
+    iw, iq, I, J = 2, 1, 0, 1
+    ref = 0
+    for it in range(6):
+        for R in rpa.r_grid(s.period):
+            ref = ref + 2.0 * g["cos_t2f"][iw, it] * np.exp(2j * np.pi * np.dot(s.kfrac[iq], R)) * chi0s[it][(I, J, R)]
+    assert np.abs(q[(iw, iq, I, J)] - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_cell2_ec_negative_and_sums():
+    s = _sys(natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0, nfreq=6)
+    r = rpa.rpa_from_system(s)
+    assert r["ec"] < 0
+    assert abs(r["contrib"].sum().real - r["ec"]) < 1e-12 * abs(r["ec"])
