@@ -1,0 +1,217 @@
+"""Benchmark: chi0 + RPA Ec wall seconds per frequency point on one MI355X.
+
+A step is one full evaluation of the space-time path (G(+-tau) for every minimax
+time point, the chi0 contractions, the tau -> omega / R -> q transform and the
+Ec log-determinants for every (q, omega)) through the C-ABI call
+librpa_get_rpa_correlation_energy, with all inputs already set (device-resident
+plan).  The workload is the 128-atom two-species cubic cell at Gamma with 16
+minimax points (synthetic seeded inputs).
+
+The timed path is the fastest contraction precision whose Ec stays within
+1e-5 Ha/atom of the plain FP64 GPU path, which is timed beside it.
+
+Output: one JSON line (metric, value = s/freq, lower is better) with a
+`roofline` object for the dominant kernel and a `cpu_baseline` object (the FP64
+oracle timed on a bounded sample of the same workload, scaled).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+HBM_GBS = 8000.0
+EC_TOL_PER_ATOM = 1e-5
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_steps(L, h, o, nk, prec, steps, warmup):
+    h.set_precision(prec, prec)
+    ec = None
+    for _ in range(warmup):
+        ec, _ = h.get_rpa_correlation_energy(o, nk)
+    L.profile_reset()
+    times = []
+    for _ in range(steps):
+        _sync()
+        t0 = time.perf_counter()
+        ec, contrib = h.get_rpa_correlation_energy(o, nk)
+        _sync()
+        times.append(time.perf_counter() - t0)
+    return ec, times, L.profile(), h.get_stats()
+
+
+def dominant_kernel(prof, prec_name):
+    """Largest-time region among the single-kernel GEMM regions."""
+    cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_TB", "chi0_contract",
+                                                            "ec_gemm"))}
+    name, (ms, calls, flops, _bytes) = max(cands.items(), key=lambda kv: kv[1][0])
+    ach = flops / (ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[prec_name if name.startswith("chi0") else prec_name]
+    return name, dict(bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s", frac=round(ach / peak, 4),
+                      traffic=None, kernel=name, calls=int(calls), avg_ms=round(ms / calls, 4))
+
+
+def cpu_baseline(sys_, grid, nfreq):
+    """FP64 oracle timed on a bounded sample of the workload and scaled to s/freq."""
+    from oracle import rpa
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cs = rpa.fold_cs(sys_)
+    Rs = rpa.r_grid(sys_.period)
+    it = len(grid["tau"]) // 2
+    tau = grid["tau"][it]
+    t0 = time.perf_counter()
+    Gp = rpa.gf_R(sys_, tau, Rs)
+    Gn = rpa.gf_R(sys_, -tau, Rs)
+    t_g = time.perf_counter() - t0
+    J = sys_.natom // 2
+    nI = 3
+    t0 = time.perf_counter()
+    rpa.chi0s_factorized(sys_, cs, Gp, Gn, [(0, J)], Rs)
+    t_one = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rpa.chi0s_factorized(sys_, cs, Gp, Gn, [(I, J) for I in range(nI)], Rs)
+    t_three = time.perf_counter() - t0
+    t_out = max((t_three - t_one) / (nI - 1), 1e-9)
+    t_w = max(t_one - t_out, 0.0)
+    n_out = sys_.natom * (sys_.natom + 1) // 2 * len(Rs)
+    chi0_per_tau = t_g + sys_.natom * t_w + n_out * t_out
+    # Ec: one omega at a quarter-size matrix, scaled by n^3
+    n = int(sys_.nabf.sum())
+    m = max(n // 4, 64)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal((m, m))
+    V = b @ b.T / m + 1e-3 * np.eye(m)
+    x = rng.standard_normal((m, m))
+    chi = -(x @ x.T) / m
+    t0 = time.perf_counter()
+    Pi = V @ chi
+    rpa.ln_det_ref(np.eye(m) - Pi)
+    t_ec1 = (time.perf_counter() - t0) * (n / m) ** 3
+    total = chi0_per_tau * len(grid["tau"]) + t_ec1 * nfreq * sys_.nk
+    return dict(value=round(total / nfreq, 3), unit="s/freq", cores=threads, kind="port",
+                sample=(f"numpy FP64 oracle: G(+-tau) and W(J={J}) plus {nI} output blocks at tau {it}, "
+                        f"scaled to {n_out} blocks x {len(grid['tau'])} tau; Ec: one omega at n={m} scaled "
+                        f"by (n_abf/{m})^3; total / nfreq"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cubic128")
+    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
+              "(no inter-GPU sharding in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import librpa_amd as L
+    from librpa_amd import synth
+
+    L.init_global()
+    t0 = time.time()
+    dev = None
+    try:
+        import torch
+        dev = "cuda" if torch.cuda.is_available() else None
+    except Exception:
+        pass
+    make = synth.WORKLOADS[args.workload]
+    try:
+        s = make(vq_device=dev)
+    except TypeError:
+        s = make()
+    _log(f"workload {args.workload}: {s.natom} atoms, n_ao {s.nbasis}, n_abf {s.nabf_tot}, nk {s.nk}, "
+         f"nfreq {s.nfreq}, Cs blocks {len(s.cs)} (generated in {time.time() - t0:.1f}s)")
+    h = L.Handler()
+    t0 = time.time()
+    L.load_system(h, s)
+    _log(f"inputs set in {time.time() - t0:.1f}s")
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    grid = h.get_tfgrids(o)
+
+    # plain FP64 GPU path
+    ec64, t64, prof64, st64 = run_steps(L, h, o, s.nk, L.FP64, 1, 1)
+    s64 = float(np.median(t64))
+    _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step  chi0 {st64['t_chi0']:.3f}s  Ec {st64['t_ec']:.3f}s")
+
+    cand = ["bf16x3", "fp32"] if args.prec == "auto" else [args.prec]
+    chosen = None
+    for pn in cand:
+        if pn == "fp64":
+            ec, times, prof, st = ec64, t64, prof64, st64
+            if args.steps > 1 or args.warmup > 0:
+                ec, times, prof, st = run_steps(L, h, o, s.nk, L.FP64, args.steps, args.warmup)
+        else:
+            ec, times, prof, st = run_steps(L, h, o, s.nk, L.PREC[pn], args.steps, args.warmup)
+        dec = abs(ec - ec64) / s.natom
+        _log(f"{pn}: Ec {ec:.10f}  dEc/atom {dec:.3e}  {np.median(times):.3f} s/step")
+        if dec <= EC_TOL_PER_ATOM or args.prec != "auto":
+            chosen = (pn, ec, times, prof, st, dec)
+            break
+    if chosen is None:
+        ec, times, prof, st = run_steps(L, h, o, s.nk, L.FP64, args.steps, args.warmup)
+        chosen = ("fp64", ec, times, prof, st, 0.0)
+    pn, ec, times, prof, st, dec = chosen
+    step = float(np.median(times))
+    total_flops = st["flops_chi0_alg"]
+    kname, roof = dominant_kernel(prof, pn)
+    res = {
+        "metric": "chi0+RPA Ec wall s/freq",
+        "value": round(step / s.nfreq, 5),
+        "unit": "s/freq",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": pn,
+        "data": "synthetic (seeded generator, librpa_amd/synth.py)",
+        "config": {"workload": args.workload, "atoms": s.natom, "n_ao": s.nbasis, "n_abf": s.nabf_tot,
+                   "nk": s.nk, "nfreq": s.nfreq, "parallelism": "single GPU"},
+        "ec_ha": ec,
+        "ec_fp64_ha": ec64,
+        "dEc_per_atom_ha": dec,
+        "fp64_s_per_freq": round(s64 / s.nfreq, 5),
+        "speedup_vs_fp64_gpu": round(s64 / step, 3),
+        "chi0_tflops_effective": round(total_flops / st["t_chi0"] / 1e12, 3) if st["t_chi0"] > 0 else None,
+        "t_chi0_s": round(st["t_chi0"], 4),
+        "t_ec_s": round(st["t_ec"], 4),
+        "roofline": roof,
+        "regions_ms": {k: round(v[0] / max(1, args.steps), 3) for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        res["cpu_baseline"] = cpu_baseline(s, grid, s.nfreq)
+        _log(f"cpu baseline sampled in {time.time() - t0:.1f}s")
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -458,8 +458,15 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             const auto &o = *p_opts;
             if (o.use_shrink_abfs == LIBRPA_SWITCH_ON) LRA_THROW("ABF shrinking is outside this build's path");
             prepare_grid(d, o);
-            auto eng = std::make_shared<RpaEngine>();
-            eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
+            // the tau-invariant plan (packed Cs, problem lists, G-build operands) is reused
+            // until a setter changes the inputs
+            auto eng = d.rpa;
+            if (!eng || eng->prec != d.chi0_prec || eng->planned_ntau != d.grid.n) {
+                d.rpa.reset();
+                eng = std::make_shared<RpaEngine>();
+                eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
+            }
+            eng->ec_prec = d.ec_prec;
             eng->build_chi0(d, d.grid, o.n_bands_chi0);
             auto r = eng->ec(d, d.grid);
             if ((int)r.qcontrib.size() != n_ibz_kpoints && o.output_level >= LIBRPA_VERBOSE_WARN)
@@ -477,8 +484,7 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             d.stats[5] = (double)eng->n_cs_blocks;
             d.stats[6] = eng->NABF;
             d.stats[7] = eng->NAO;
-            if (d.keep_chi0) d.rpa = eng;
-            else d.rpa.reset();  // compute_rpa.cpp:164 frees chi0 after Ec
+            d.rpa = eng;
             return r.ec_re;
         },
         std::nan(""));

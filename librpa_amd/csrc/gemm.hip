@@ -228,13 +228,15 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
         }
     }
 
-    // epilogue: acc[a][b][r] = C[m0 + wm*WTM + a*16 + 4*(lane/16) + r][n0 + wn*WTN + b*16 + lane%16]
+    // epilogue: acc[a][b][r] holds C[row][n0 + wn*WTN + b*16 + lane%16]
     const bool split = P.ncb < P.N;
 #pragma unroll
     for (int a = 0; a < FM; ++a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gm = m0 + wm * WTM + a * 16 + 4 * kq + r;
+            // f64 16x16x4 places register r of lane group kq at row 4r + kq, the f32 / bf16
+            // variants at row 4kq + r
+            const int gm = m0 + wm * WTM + a * 16 + (MODE == 0 ? 4 * r + kq : 4 * kq + r);
             if (gm >= P.M) continue;
             const long long crow = rt ? rowtab[2ll * (P.rt_off + gm) + 1] : P.c_off + (long long)gm * P.ldc;
 #pragma unroll

@@ -52,9 +52,10 @@ void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStr
 void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s);
 void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s);
 
-// Blocked Cholesky (lower, row-major, FP64): A = L L^T in place.
-// logdet (device scalar) += 2 sum log L_jj; *flag set to 1 on a non-positive pivot.
-// When keep_L is false only the log-determinant is produced.
+// Blocked Cholesky with signed pivots (lower, row-major, FP64): A = L S L^T in place,
+// S = diag(+-1); equal to Cholesky for positive-definite A.  logdet (device scalar)
+// += ln |det A|; *flag set to 1 on a zero / NaN pivot.  When keep_L is false only the
+// log-determinant is produced.
 struct CholWork {
     DevBuf linv, panel;
 };

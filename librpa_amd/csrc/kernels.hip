@@ -293,12 +293,15 @@ void launch_copy_block(const double *src, int lds, double *dst, int ldd, int row
 // ------------------------------------------------------------------------------------
 constexpr int CHOL_NB = 64;
 
-// Factor the nb x nb diagonal block at (j0, j0) in LDS, write L11 back, write
-// Linv = L11^{-1} (row-major, CHOL_NB pitch) and accumulate 2 sum log L_jj.
+// Factor the nb x nb diagonal block at (j0, j0) in LDS as L S L^T with S = diag(+-1)
+// (a Cholesky factorisation that tolerates negative pivots), write L11 back, write
+// Linv = L11^{-1} (row-major, CHOL_NB pitch) and the pivot signs, and accumulate
+// sum log|d_j| = ln |det| of the block.  A zero or NaN pivot sets *flag.
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int j0, int nb, double *Linv,
-                                                         double *logdet, int *flag) {
+                                                         double *sgn, double *logdet, int *flag) {
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
     __shared__ double X[CHOL_NB][CHOL_NB + 1];
+    __shared__ double S[CHOL_NB];
     __shared__ int bad;
     const int t = threadIdx.x;
     if (t == 0) bad = 0;
@@ -310,17 +313,18 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int
     for (int j = 0; j < nb; ++j) {
         if (t == 0) {
             double d = L[j][j];
-            if (!(d > 0.0)) { bad = 1; d = 1.0; }
-            L[j][j] = sqrt(d);
+            if (!(fabs(d) > 0.0)) { bad = 1; d = 1.0; }
+            S[j] = d > 0 ? 1.0 : -1.0;
+            L[j][j] = sqrt(fabs(d));
         }
         __syncthreads();
-        const double djj = L[j][j];
+        const double djj = L[j][j], sj = S[j];
         for (int i = j + 1 + t; i < nb; i += blockDim.x) L[i][j] /= djj;
         __syncthreads();
         const int m = nb - j - 1;
         for (int e = t; e < m * m; e += blockDim.x) {
             const int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) L[i][k] -= L[i][j] * L[k][j];
+            if (k <= i) L[i][k] -= sj * L[i][j] * L[k][j];
         }
         __syncthreads();
     }
@@ -343,12 +347,21 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int
         const int i = e / CHOL_NB, j = e % CHOL_NB;
         Linv[e] = (i < nb && j < nb) ? X[i][j] : 0.0;
     }
+    for (int e = t; e < CHOL_NB; e += blockDim.x) sgn[e] = e < nb ? S[e] : 1.0;
     if (t == 0) {
         double s = 0;
         for (int j = 0; j < nb; ++j) s += 2.0 * log(L[j][j]);
         *logdet += s;
         if (bad) *flag = 1;
     }
+}
+
+// Qs[r][c] = Q[r][c] * sgn[c]
+__global__ void scale_cols_kernel(const double *__restrict__ Q, double *__restrict__ Qs, const double *__restrict__ sgn,
+                                  int m, int nb) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)m * nb) return;
+    Qs[idx] = Q[idx] * sgn[idx % nb];
 }
 
 struct CholPlan {
@@ -362,8 +375,12 @@ static std::map<std::tuple<int, int, int, hipStream_t>, CholPlan> &chol_plans() 
 }
 
 void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
-    w.linv.reserve(sizeof(double) * CHOL_NB * CHOL_NB);
-    w.panel.reserve(sizeof(double) * (size_t)n * CHOL_NB);
+    w.linv.reserve(sizeof(double) * (CHOL_NB * CHOL_NB + CHOL_NB));
+    w.panel.reserve(sizeof(double) * (size_t)n * CHOL_NB * 2);
+    double *Linv = w.linv.as<double>();
+    double *sgn = Linv + CHOL_NB * CHOL_NB;
+    double *Q = w.panel.as<double>();
+    double *Qs = Q + (size_t)n * CHOL_NB;
     auto key = std::make_tuple(n, lda, 0, s);
     auto &plans = chol_plans();
     auto it = plans.find(key);
@@ -376,12 +393,12 @@ void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool 
             cp.nbs.push_back(nb);
             GemmBatch tr, sy;
             if (m > 0) {
-                // P = A21 * Linv^T  (A21: m x nb at (j0+nb, j0))
+                // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)); L21 = Q S
                 GemmProb p = make_prob(m, nb, nb, (long long)(j0 + nb) * lda + j0, lda, 0, CHOL_NB, 0, nb);
                 tr.trans_b = true;
                 tr.add(p);
                 tr.finalize(T64x64, s);
-                // A22 -= P P^T (lower tiles)
+                // A22 -= (Q S) Q^T (lower tiles)
                 GemmProb q = make_prob(m, m, nb, 0, nb, 0, nb, (long long)(j0 + nb) * lda + (j0 + nb), lda);
                 sy.trans_b = true;
                 sy.lower_only = true;
@@ -396,14 +413,16 @@ void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool 
     CholPlan &cp = it->second;
     for (size_t b = 0; b < cp.j0s.size(); ++b) {
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, A, lda, j0, nb, w.linv.as<double>(), logdet,
-                           flag);
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, A, lda, j0, nb, Linv, sgn, logdet, flag);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
         if (m <= 0) break;
-        gemm_run(Prec::F64, cp.trsm[b], A, w.linv.get(), w.panel.get(), 1.0, 0.0, s);
-        if (keep_L) launch_copy_block(w.panel.as<double>(), nb, A + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
-        gemm_run(Prec::F64, cp.syrk[b], w.panel.get(), w.panel.get(), A, -1.0, 1.0, s);
+        gemm_run(Prec::F64, cp.trsm[b], A, Linv, Q, 1.0, 0.0, s);
+        hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256)), dim3(256), 0, s, Q, Qs, sgn, m,
+                           nb);
+        HIP_CHECK(hipGetLastError());
+        if (keep_L) launch_copy_block(Qs, nb, A + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
+        gemm_run(Prec::F64, cp.syrk[b], Qs, Q, A, -1.0, 1.0, s);
     }
     if (keep_L) launch_zero_upper<double>(A, n, lda, s);
 }

@@ -50,6 +50,7 @@ int RpaEngine::fold_index(int x, int y, int z) const {
 void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     prec = p;
     ec_prec = pe;
+    planned_ntau = ntau;
     ts = (prec == Prec::F64) ? 8 : 4;
     const auto &mf = ds.mf;
     natom = (int)ds.nb_wfc.size();
@@ -470,10 +471,12 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 prof.begin("chi0_W", stream);
                 gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
                 prof.end("chi0_W", stream, c.wb.flops);
-                prof.begin("chi0_T", stream);
+                prof.begin("chi0_TB", stream);
                 gemm_run(prec, c.tb, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                prof.end("chi0_TB", stream, c.tb.flops);
+                prof.begin("chi0_TA", stream);
                 gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 1.0, stream);
-                prof.end("chi0_T", stream, c.tb.flops + c.ta.flops);
+                prof.end("chi0_TA", stream, c.ta.flops);
                 prof.begin("chi0_contract", stream);
                 gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
                 prof.end("chi0_contract", stream, c.fin.flops);
@@ -610,12 +613,14 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
                 const int iw = w0 + w;
                 char *Dw = dD.as<char>() + (size_t)w * nn * es;
                 double *acc = dacc.as<double>() + 2 * ((size_t)iq * nw + iw);
-                prof.begin("ec_gemm", stream);
+                prof.begin("ec_gemm_x", stream);
                 gemm_run(ec_prec, bx, Dw, Lp, dX.get(), 1.0, 0.0, stream);
+                prof.end("ec_gemm_x", stream, bx.flops);
                 if (f64) launch_set_identity<double>(dM.as<double>(), n2, n2, stream);
                 else launch_set_identity<float>(dM.as<float>(), n2, n2, stream);
+                prof.begin("ec_gemm_m", stream);
                 gemm_run(ec_prec, bm, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
-                prof.end("ec_gemm", stream, bx.flops + bm.flops * 0.5);
+                prof.end("ec_gemm_m", stream, bm.flops * 0.5);
                 double *M64 = dM.as<double>();
                 if (!f64) {
                     launch_copy_lower_f2d(dM.as<float>(), dM64.as<double>(), n2, n2, stream);
@@ -633,8 +638,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     HIP_CHECK(hipMemcpyAsync(hacc.data(), dacc.get(), hacc.size() * 8, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipMemcpyAsync(hflag, dflag, sizeof(hflag), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
-    if (hflag[1]) LRA_THROW("Coulomb matrix is not positive definite (Cholesky pivot <= 0)");
-    if (hflag[0]) LRA_THROW("1 - V chi0 is not positive definite at some (q, omega): chi0 is not negative semi-definite");
+    if (hflag[1]) LRA_THROW("Coulomb matrix is singular or not finite (zero pivot in its factorisation)");
+    if (hflag[0]) LRA_THROW("1 - V chi0 is singular or not finite at some (q, omega)");
     const double half = embed ? 0.5 : 1.0;
     for (int iq = 0; iq < nq; ++iq)
         for (int iw = 0; iw < nw; ++iw) {

@@ -69,6 +69,7 @@ struct RpaEngine {
     double flops_chi0_exec = 0;
     double t_chi0 = 0, t_ec = 0;
     int ntau_done = 0;
+    int planned_ntau = 0;
 
     void plan(const Dataset &ds, Prec p, Prec pe, int ntau);
     void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);

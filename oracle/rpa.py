@@ -98,7 +98,7 @@ def gf_R(sys, tau, Rs, nbands_g=-1):
     """Dense G(R, tau) (nbasis x nbasis, real) for each R in Rs."""
     s = occ_weights(sys, tau, nbands_g)
     U = sys.wfc[0]
-    M = np.einsum('kni,kn,knj->kij', U, s, U.conj())
+    M = np.stack([(U[k].T * s[k][None, :]) @ U[k].conj() for k in range(sys.nk)])
     out = {}
     for R in Rs:
         ph = np.exp(-2j * np.pi * (sys.kfrac @ np.asarray(R, dtype=np.float64)))
@@ -155,7 +155,7 @@ def chi0s_factorized(sys, cs, Gp, Gn, pairs, Rs):
                 acc = np.zeros((ao.total, int(sys.nao[J]), int(sys.nabf[J])))
                 for (L, R2) in nb.get(J, []):
                     g = G[fold_R(Sv + np.array(R2), per)][:, ao.sl(L)]
-                    acc += np.einsum('xl,bjl->xjb', g, cs[(J, L, R2)])
+                    acc += np.tensordot(g, cs[(J, L, R2)], axes=([1], [2])).transpose(0, 2, 1)
                 W[(sgn, J, S)] = acc
     out = {}
     for (I, J) in pairs:
@@ -172,9 +172,12 @@ def chi0s_factorized(sys, cs, Gp, Gn, pairs, Rs):
                 Wn_KJ = W[('-', J, S)][ao.sl(K)]
                 gp_kj = _gblk(Gp, ao, K, J, Rv - np.array(R1), per)
                 gn_kj = _gblk(Gn, ao, K, J, Rv - np.array(R1), per)
-                T = (np.einsum('ij,kjb->kib', gp_ij, Wn_KJ) + np.einsum('ij,kjb->kib', gn_ij, Wp_KJ)
-                     + np.einsum('kj,ijb->kib', gn_kj, Wp_IJ) + np.einsum('kj,ijb->kib', gp_kj, Wn_IJ))
-                acc += np.einsum('aik,kib->ab', cs[(I, K, R1)], T)
+                # T[i, k, nu] = sum_j G_IJ[i, j] W_KJ[k, j, nu] + G_KJ[k, j] W_IJ[i, j, nu]
+                TA = np.matmul(gp_ij[None], Wn_KJ) + np.matmul(gn_ij[None], Wp_KJ)   # (k, i, nu)
+                TB = np.matmul(gn_kj[None], Wp_IJ) + np.matmul(gp_kj[None], Wn_IJ)   # (i, k, nu)
+                T = TA.transpose(1, 0, 2) + TB
+                c = cs[(I, K, R1)]  # (mu, i, k)
+                acc += c.reshape(c.shape[0], -1) @ T.reshape(-1, T.shape[2])
             out[(I, J, R)] = acc
     return out
 
