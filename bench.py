@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--workload", default="cubic128")
     ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
+                    help="where the synthetic Coulomb matrix product is formed (cpu under profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
@@ -133,11 +135,12 @@ def main():
     L.init_global()
     t0 = time.time()
     dev = None
-    try:
-        import torch
-        dev = "cuda" if torch.cuda.is_available() else None
-    except Exception:
-        pass
+    if args.gen_device == "auto":
+        try:
+            import torch
+            dev = "cuda" if torch.cuda.is_available() else None
+        except Exception:
+            pass
     make = synth.WORKLOADS[args.workload]
     try:
         s = make(vq_device=dev)

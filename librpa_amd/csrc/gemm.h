@@ -32,6 +32,7 @@ struct GemmProb {
     long long a_off, b_off, c_off;
     int lda, ldb, ldc;
     int rt_off;    // offset into the row table, -1 if not row-indirect
+    int kmode;     // triangular operand: 0 none, 1 B lower (k < n is zero), 2 A = L^T (k < m is zero)
 };
 
 // Tile configurations (BM x BN, 256 threads).

@@ -165,8 +165,12 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
     const int rowB = wn * WTN + (lane & 15);
     const int kq = lane >> 4;
 
-    gload(0);
-    for (int k0 = 0; k0 < P.K; k0 += BK) {
+    // a triangular operand lets the K loop start at the tile's diagonal
+    int kbeg = 0;
+    if (P.kmode == 1) kbeg = (n0 / BK) * BK;
+    else if (P.kmode == 2) kbeg = (m0 / BK) * BK;
+    gload(kbeg);
+    for (int k0 = kbeg; k0 < P.K; k0 += BK) {
         __syncthreads();
         sstore();
         __syncthreads();

@@ -61,5 +61,9 @@ struct CholWork {
 };
 void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w,
                      hipStream_t s);
+// The same for nbatch matrices at A + b * strideA, all factorised in the same launches;
+// matrix b adds its ln|det| to logdet[b * ld_stride].
+void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
+                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s);
 
 }  // namespace lra

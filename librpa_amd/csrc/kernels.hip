@@ -293,22 +293,29 @@ void launch_copy_block(const double *src, int lds, double *dst, int ldd, int row
 // ------------------------------------------------------------------------------------
 constexpr int CHOL_NB = 64;
 
-// Factor the nb x nb diagonal block at (j0, j0) in LDS as L S L^T with S = diag(+-1)
-// (a Cholesky factorisation that tolerates negative pivots), write L11 back, write
-// Linv = L11^{-1} (row-major, CHOL_NB pitch) and the pivot signs, and accumulate
-// sum log|d_j| = ln |det| of the block.  A zero or NaN pivot sets *flag.
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int j0, int nb, double *Linv,
-                                                         double *sgn, double *logdet, int *flag) {
+// Factor the nb x nb diagonal block at (j0, j0) of matrix blockIdx.x in LDS as
+// L S L^T with S = diag(+-1) (a Cholesky factorisation that tolerates negative
+// pivots), write L11 back, write Linv = L11^{-1} (row-major, CHOL_NB pitch) and the
+// pivot signs, and accumulate sum log|d_j| = ln |det| of the block.  A zero or NaN
+// pivot sets *flag.  Thread (ty, tx) of a 16 x 16 layout owns rows ty + 16 a and
+// columns tx + 16 b of the block.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long strideA, int lda, int j0, int nb,
+                                                         double *work, double *logdet0, long long ld_stride,
+                                                         int *flag) {
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
     __shared__ double X[CHOL_NB][CHOL_NB + 1];
     __shared__ double S[CHOL_NB];
     __shared__ int bad;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    double *A = A0 + strideA * blockIdx.x;
+    double *Linv = work + (long long)blockIdx.x * (CHOL_NB * CHOL_NB + CHOL_NB);
+    double *sgn = Linv + CHOL_NB * CHOL_NB;
     if (t == 0) bad = 0;
-    for (int e = t; e < nb * nb; e += blockDim.x) {
-        const int i = e / nb, j = e % nb;
-        L[i][j] = A[(long long)(j0 + i) * lda + j0 + j];
-    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            const int i = ty + 16 * a, k = tx + 16 * b;
+            L[i][k] = (i < nb && k < nb) ? A[(long long)(j0 + i) * lda + j0 + k] : (i == k ? 1.0 : 0.0);
+        }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
         if (t == 0) {
@@ -318,24 +325,29 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int
             L[j][j] = sqrt(fabs(d));
         }
         __syncthreads();
-        const double djj = L[j][j], sj = S[j];
-        for (int i = j + 1 + t; i < nb; i += blockDim.x) L[i][j] /= djj;
+        if (t > j && t < nb) L[t][j] /= L[j][j];
         __syncthreads();
-        const int m = nb - j - 1;
-        for (int e = t; e < m * m; e += blockDim.x) {
-            const int i = j + 1 + e / m, k = j + 1 + e % m;
-            if (k <= i) L[i][k] -= sj * L[i][j] * L[k][j];
+        const double sj = S[j];
+        for (int a = 0; a < 4; ++a) {
+            const int i = ty + 16 * a;
+            if (i <= j) continue;
+            const double lij = sj * L[i][j];
+            for (int b = 0; b < 4; ++b) {
+                const int k = tx + 16 * b;
+                if (k > j && k <= i) L[i][k] -= lij * L[k][j];
+            }
         }
         __syncthreads();
     }
-    for (int e = t; e < nb * nb; e += blockDim.x) {
-        const int i = e / nb, j = e % nb;
-        if (j <= i) A[(long long)(j0 + i) * lda + j0 + j] = L[i][j];
-    }
-    // inverse of the lower-triangular factor, one column per thread
-    if (t < nb) {
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            const int i = ty + 16 * a, k = tx + 16 * b;
+            if (i < nb && k <= i) A[(long long)(j0 + i) * lda + j0 + k] = L[i][k];
+        }
+    // inverse of the unit-padded lower-triangular factor: column c by thread c
+    if (t < CHOL_NB) {
         const int c = t;
-        for (int r = 0; r < nb; ++r) {
+        for (int r = 0; r < CHOL_NB; ++r) {
             if (r < c) { X[r][c] = 0; continue; }
             double x = (r == c) ? 1.0 : 0.0;
             for (int k = c; k < r; ++k) x -= L[r][k] * X[k][c];
@@ -343,24 +355,28 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int lda, int
         }
     }
     __syncthreads();
-    for (int e = t; e < CHOL_NB * CHOL_NB; e += blockDim.x) {
-        const int i = e / CHOL_NB, j = e % CHOL_NB;
-        Linv[e] = (i < nb && j < nb) ? X[i][j] : 0.0;
-    }
-    for (int e = t; e < CHOL_NB; e += blockDim.x) sgn[e] = e < nb ? S[e] : 1.0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            const int i = ty + 16 * a, k = tx + 16 * b;
+            Linv[i * CHOL_NB + k] = (i < nb && k < nb) ? X[i][k] : 0.0;
+        }
+    if (t < CHOL_NB) sgn[t] = t < nb ? S[t] : 1.0;
     if (t == 0) {
-        double s = 0;
-        for (int j = 0; j < nb; ++j) s += 2.0 * log(L[j][j]);
-        *logdet += s;
+        double sum = 0;
+        for (int j = 0; j < nb; ++j) sum += 2.0 * log(L[j][j]);
+        logdet0[ld_stride * blockIdx.x] += sum;
         if (bad) *flag = 1;
     }
 }
 
-// Qs[r][c] = Q[r][c] * sgn[c]
-__global__ void scale_cols_kernel(const double *__restrict__ Q, double *__restrict__ Qs, const double *__restrict__ sgn,
-                                  int m, int nb) {
+// Qs[r][c] = Q[r][c] * sgn[c] for matrix blockIdx.y of the batch
+__global__ void scale_cols_kernel(const double *__restrict__ panel, const double *__restrict__ work, int n, int m,
+                                  int nb) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)m * nb) return;
+    const double *Q = panel + (long long)blockIdx.y * 2 * n * CHOL_NB;
+    double *Qs = const_cast<double *>(Q) + (long long)n * CHOL_NB;
+    const double *sgn = work + (long long)blockIdx.y * (CHOL_NB * CHOL_NB + CHOL_NB) + CHOL_NB * CHOL_NB;
     Qs[idx] = Q[idx] * sgn[idx % nb];
 }
 
@@ -369,19 +385,18 @@ struct CholPlan {
     std::vector<GemmBatch> trsm, syrk;
 };
 
-static std::map<std::tuple<int, int, int, hipStream_t>, CholPlan> &chol_plans() {
-    static std::map<std::tuple<int, int, int, hipStream_t>, CholPlan> p;
+static std::map<std::tuple<int, int, int, long long, hipStream_t>, CholPlan> &chol_plans() {
+    static std::map<std::tuple<int, int, int, long long, hipStream_t>, CholPlan> p;
     return p;
 }
 
-void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
-    w.linv.reserve(sizeof(double) * (CHOL_NB * CHOL_NB + CHOL_NB));
-    w.panel.reserve(sizeof(double) * (size_t)n * CHOL_NB * 2);
-    double *Linv = w.linv.as<double>();
-    double *sgn = Linv + CHOL_NB * CHOL_NB;
-    double *Q = w.panel.as<double>();
-    double *Qs = Q + (size_t)n * CHOL_NB;
-    auto key = std::make_tuple(n, lda, 0, s);
+void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
+                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
+    const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
+    const long long pstride = 2LL * n * CHOL_NB;
+    w.linv.reserve(sizeof(double) * wstride * nbatch);
+    w.panel.reserve(sizeof(double) * pstride * nbatch);
+    auto key = std::make_tuple(n, lda, nbatch, strideA, s);
     auto &plans = chol_plans();
     auto it = plans.find(key);
     if (it == plans.end()) {
@@ -393,17 +408,19 @@ void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool 
             cp.nbs.push_back(nb);
             GemmBatch tr, sy;
             if (m > 0) {
-                // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)); L21 = Q S
-                GemmProb p = make_prob(m, nb, nb, (long long)(j0 + nb) * lda + j0, lda, 0, CHOL_NB, 0, nb);
                 tr.trans_b = true;
-                tr.add(p);
-                tr.finalize(T64x64, s);
-                // A22 -= (Q S) Q^T (lower tiles)
-                GemmProb q = make_prob(m, m, nb, 0, nb, 0, nb, (long long)(j0 + nb) * lda + (j0 + nb), lda);
                 sy.trans_b = true;
                 sy.lower_only = true;
-                sy.add(q);
-                sy.finalize(m >= 2048 ? T128x128 : T64x64, s);
+                for (int b = 0; b < nbatch; ++b) {
+                    // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)); L21 = Q S
+                    tr.add(make_prob(m, nb, nb, strideA * b + (long long)(j0 + nb) * lda + j0, lda, wstride * b,
+                                     CHOL_NB, pstride * b, nb));
+                    // A22 -= (Q S) Q^T (lower tiles)
+                    sy.add(make_prob(m, m, nb, pstride * b + (long long)n * CHOL_NB, nb, pstride * b, nb,
+                                     strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
+                }
+                tr.finalize(T64x64, s);
+                sy.finalize((long long)m * nbatch >= 4096 ? T128x128 : T64x64, s);
             }
             cp.trsm.push_back(std::move(tr));
             cp.syrk.push_back(std::move(sy));
@@ -411,20 +428,31 @@ void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool 
         it = plans.emplace(key, std::move(cp)).first;
     }
     CholPlan &cp = it->second;
+    double *work = w.linv.as<double>();
+    double *panel = w.panel.as<double>();
     for (size_t b = 0; b < cp.j0s.size(); ++b) {
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, A, lda, j0, nb, Linv, sgn, logdet, flag);
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(256), 0, s, A, strideA, lda, j0, nb, work, logdet,
+                           ld_stride, flag);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
         if (m <= 0) break;
-        gemm_run(Prec::F64, cp.trsm[b], A, Linv, Q, 1.0, 0.0, s);
-        hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256)), dim3(256), 0, s, Q, Qs, sgn, m,
-                           nb);
+        gemm_run(Prec::F64, cp.trsm[b], A, work, panel, 1.0, 0.0, s);
+        hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256), nbatch), dim3(256), 0, s, panel,
+                           work, n, m, nb);
         HIP_CHECK(hipGetLastError());
-        if (keep_L) launch_copy_block(Qs, nb, A + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
-        gemm_run(Prec::F64, cp.syrk[b], Qs, Q, A, -1.0, 1.0, s);
+        if (keep_L)
+            for (int bb = 0; bb < nbatch; ++bb)
+                launch_copy_block(panel + pstride * bb + (long long)n * CHOL_NB, nb,
+                                  A + strideA * bb + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
+        gemm_run(Prec::F64, cp.syrk[b], panel, panel, A, -1.0, 1.0, s);
     }
-    if (keep_L) launch_zero_upper<double>(A, n, lda, s);
+    if (keep_L)
+        for (int bb = 0; bb < nbatch; ++bb) launch_zero_upper<double>(A + strideA * bb, n, lda, s);
+}
+
+void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
+    cholesky_logdet_batched(A, 0, 1, n, lda, logdet, 0, flag, keep_L, w, s);
 }
 
 }  // namespace lra

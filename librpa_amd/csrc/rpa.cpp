@@ -518,26 +518,47 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     res.qcontrib.assign(ibz.size(), cplx(0, 0));
     const bool f64 = ec_prec == Prec::F64;
     const size_t es = f64 ? 8 : 4;
-    const int WCH = 8;
-    DevBuf dV(nn * 8), dL32, dD(nn * es * WCH), dX(nn * es), dM(nn * es), dM64, dacc;
-    if (!f64) { dL32.alloc(nn * 4); dM64.alloc(nn * 8); }
     const int nw = g.n;
     const int nq = (int)ibz.size();
+    // all omega of a q are factorised together, as far as memory allows
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const double per_w = (double)nn * (3 * es + (f64 ? 0 : 8));
+    const double budget = std::min(96e9, 0.6 * (double)free_b) - (double)nn * 12;
+    const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
+    DevBuf dV(nn * 8), dL32, dD(nn * es * WCH), dX(nn * es * WCH), dM(nn * es * WCH), dM64, dacc;
+    if (!f64) { dL32.alloc(nn * 4); dM64.alloc(nn * 8 * WCH); }
     dacc.alloc(sizeof(double) * 2 * nq * nw + 64);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
     int *dflag = reinterpret_cast<int *>(dacc.as<char>() + sizeof(double) * 2 * nq * nw);
     CholWork cw;
-    // GEMM batches: X = D L ; M = I - L^T X (lower tiles)
-    GemmBatch bx, bm;
-    bx.add(make_prob(n2, n2, n2, 0, n2, 0, n2, 0, n2));
-    bx.finalize(n2 >= 1024 ? T128x128 : T64x64, stream);
-    bm.trans_a = true;
-    bm.lower_only = true;
-    bm.add(make_prob(n2, n2, n2, 0, n2, 0, n2, 0, n2));
-    bm.finalize(n2 >= 1024 ? T128x128 : T64x64, stream);
     std::vector<double> gp;
     DevBuf dgp;
     std::vector<double> Vh(nn);
+    std::map<int, std::pair<GemmBatch, GemmBatch>> batches;  // chunk size -> (X = D L, M = I - L^T X)
+    auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
+        auto it = batches.find(cnt);
+        if (it != batches.end()) return it->second;
+        auto &pr = batches[cnt];
+        GemmBatch &bx = pr.first, &bm = pr.second;
+        bm.trans_a = true;
+        bm.lower_only = true;
+        for (int w = 0; w < cnt; ++w) {
+            GemmProb px = make_prob(n2, n2, n2, (long long)w * nn, n2, 0, n2, (long long)w * nn, n2);
+            px.kmode = 1;  // L is lower triangular: column tile n0 needs k >= n0 only
+            bx.add(px);
+            GemmProb pm = make_prob(n2, n2, n2, 0, n2, (long long)w * nn, n2, (long long)w * nn, n2);
+            pm.kmode = 2;  // A = L^T: row tile m0 needs k >= m0 only
+            bm.add(pm);
+        }
+        const int cfg = n2 >= 1024 ? T128x128 : T64x64;
+        bx.finalize(cfg, stream);
+        bm.finalize(cfg, stream);
+        // executed flops of the triangular products
+        bx.flops = 2.0 * cnt * (double)n2 * n2 * n2 / 2.0;
+        bm.flops = 2.0 * cnt * (double)n2 * n2 * n2 / 6.0;
+        return pr;
+    };
     for (int iq = 0; iq < nq; ++iq) {
         const int kq = ibz[iq];
         // dense V(q) from upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
@@ -578,59 +599,70 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         }
         const double *kf = &ds.kfrac[(size_t)kq * 3];
         for (int w0 = 0; w0 < nw; w0 += WCH) {
-            const int cw_n = std::min(WCH, nw - w0);
-            gp.assign((size_t)cw_n * nR * ntau_done * 2, 0.0);
-            for (int w = 0; w < cw_n; ++w)
-                for (int r = 0; r < nR; ++r) {
-                    const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
-                    for (int t = 0; t < ntau_done; ++t) {
-                        const double gam = g.cos_t2f[(size_t)(w0 + w) * g.n + t] * 2.0 / nspin;
-                        gp[(((size_t)w * nR + r) * ntau_done + t) * 2] = gam * std::cos(ang);
-                        gp[(((size_t)w * nR + r) * ntau_done + t) * 2 + 1] = gam * std::sin(ang);
-                    }
-                }
-            dgp = upload(gp, stream);
+            const int cnt = std::min(WCH, nw - w0);
+            // tau -> omega and R -> q of chi0s into dense matrices, at most 8 omega per launch
             prof.begin("chi0_transform", stream);
-            const double tbytes = (double)ts * E_tot * ntau_done + (double)es * cw_n * (double)E_tot / nR * (embed ? 4 : 1) * 2;
-            if (prec == Prec::F64 && f64)
-                launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                     (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
-                                                     dgp.as<double>(), cw_n, dD.as<double>(), (long long)nn, n, embed, stream);
-            else if (prec == Prec::F64)
-                launch_chi0_assemble<double, float>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                    (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
-                                                    dgp.as<double>(), cw_n, dD.as<float>(), (long long)nn, n, embed, stream);
-            else if (f64)
-                launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                    (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
-                                                    dgp.as<double>(), cw_n, dD.as<double>(), (long long)nn, n, embed, stream);
-            else
-                launch_chi0_assemble<float, float>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                   (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
-                                                   dgp.as<double>(), cw_n, dD.as<float>(), (long long)nn, n, embed, stream);
-            prof.end("chi0_transform", stream, 0, tbytes);
-            for (int w = 0; w < cw_n; ++w) {
-                const int iw = w0 + w;
-                char *Dw = dD.as<char>() + (size_t)w * nn * es;
-                double *acc = dacc.as<double>() + 2 * ((size_t)iq * nw + iw);
-                prof.begin("ec_gemm_x", stream);
-                gemm_run(ec_prec, bx, Dw, Lp, dX.get(), 1.0, 0.0, stream);
-                prof.end("ec_gemm_x", stream, bx.flops);
-                if (f64) launch_set_identity<double>(dM.as<double>(), n2, n2, stream);
-                else launch_set_identity<float>(dM.as<float>(), n2, n2, stream);
-                prof.begin("ec_gemm_m", stream);
-                gemm_run(ec_prec, bm, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
-                prof.end("ec_gemm_m", stream, bm.flops * 0.5);
-                double *M64 = dM.as<double>();
-                if (!f64) {
-                    launch_copy_lower_f2d(dM.as<float>(), dM64.as<double>(), n2, n2, stream);
-                    M64 = dM64.as<double>();
-                }
-                prof.begin("ec_logdet", stream);
-                launch_trace_acc(M64, n2, n2, acc, stream);
-                cholesky_logdet(M64, n2, n2, acc + 1, dflag, false, cw, stream);
-                prof.end("ec_logdet", stream, (double)n2 * n2 * n2 / 3.0);
+            for (int a0 = 0; a0 < cnt; a0 += 8) {
+                const int cw_n = std::min(8, cnt - a0);
+                gp.assign((size_t)cw_n * nR * ntau_done * 2, 0.0);
+                for (int w = 0; w < cw_n; ++w)
+                    for (int r = 0; r < nR; ++r) {
+                        const double ang =
+                            2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+                        for (int t = 0; t < ntau_done; ++t) {
+                            const double gam = g.cos_t2f[(size_t)(w0 + a0 + w) * g.n + t] * 2.0 / nspin;
+                            gp[(((size_t)w * nR + r) * ntau_done + t) * 2] = gam * std::cos(ang);
+                            gp[(((size_t)w * nR + r) * ntau_done + t) * 2 + 1] = gam * std::sin(ang);
+                        }
+                    }
+                dgp = upload(gp, stream);
+                char *Dd = dD.as<char>() + (size_t)a0 * nn * es;
+                if (prec == Prec::F64 && f64)
+                    launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                         (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
+                                                         nR, dgp.as<double>(), cw_n, (double *)Dd, (long long)nn, n,
+                                                         embed, stream);
+                else if (prec == Prec::F64)
+                    launch_chi0_assemble<double, float>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                        (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
+                                                        nR, dgp.as<double>(), cw_n, (float *)Dd, (long long)nn, n,
+                                                        embed, stream);
+                else if (f64)
+                    launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                        (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
+                                                        nR, dgp.as<double>(), cw_n, (double *)Dd, (long long)nn, n,
+                                                        embed, stream);
+                else
+                    launch_chi0_assemble<float, float>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                       (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
+                                                       nR, dgp.as<double>(), cw_n, (float *)Dd, (long long)nn, n,
+                                                       embed, stream);
             }
+            const double tbytes = (double)ts * E_tot * ntau_done * ceil_div(cnt, 8) + (double)es * cnt * nn;
+            prof.end("chi0_transform", stream, 0, tbytes);
+            auto &bb = get_batches(cnt);
+            prof.begin("ec_gemm_x", stream);
+            gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
+            prof.end("ec_gemm_x", stream, bb.first.flops);
+            for (int w = 0; w < cnt; ++w) {
+                if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream);
+                else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream);
+            }
+            prof.begin("ec_gemm_m", stream);
+            gemm_run(ec_prec, bb.second, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
+            prof.end("ec_gemm_m", stream, bb.second.flops);
+            double *M64 = dM.as<double>();
+            if (!f64) {
+                for (int w = 0; w < cnt; ++w)
+                    launch_copy_lower_f2d(dM.as<float>() + (size_t)w * nn, dM64.as<double>() + (size_t)w * nn, n2, n2,
+                                          stream);
+                M64 = dM64.as<double>();
+            }
+            double *acc0 = dacc.as<double>() + 2 * ((size_t)iq * nw + w0);
+            prof.begin("ec_logdet", stream);
+            for (int w = 0; w < cnt; ++w) launch_trace_acc(M64 + (size_t)w * nn, n2, n2, acc0 + 2 * w, stream);
+            cholesky_logdet_batched(M64, (long long)nn, cnt, n2, n2, acc0 + 1, 2, dflag, false, cw, stream);
+            prof.end("ec_logdet", stream, (double)cnt * n2 * n2 * n2 / 3.0);
         }
     }
     std::vector<double> hacc(2 * (size_t)nq * nw);
