@@ -53,6 +53,18 @@ void librpa_amd_profile_reset(void);
  * Returns the number of entries written. */
 int librpa_amd_get_stats(LibrpaHandler *h, double *out, int n);
 
+/* G0W0 intermediates after librpa_build_g0w0_sigma.  W(R, tau) block
+ * (naux_I x naux_J, row-major) of minimax time itau for the BvK cell R, all
+ * ordered atom pairs.  Returns 0 on success. */
+int librpa_amd_get_wc_block(LibrpaHandler *h, int itau, int I, int J, const int R[3], double *out);
+/* Sigma_c(R, i omega) block (nbasis_I x nbasis_J, row-major re / im) of spin
+ * ispin, frequency ifreq and BvK cell R.  Returns 0 on success. */
+int librpa_amd_get_sigma_block(LibrpaHandler *h, int ispin, int ifreq, int I, int J, const int R[3], double *re,
+                               double *im);
+/* Diagonal Kohn-Sham matrix element Sigma_c(k, i omega)_nn of state i_state at
+ * k point ik for all nfreq frequencies (ascending).  Returns nfreq, < 0 on error. */
+int librpa_amd_get_sigc_ks(LibrpaHandler *h, int ispin, int ik, int i_state, double *re, double *im);
+
 /* One dense GEMM C = alpha op(A) op(B) + beta C on host arrays through the
  * device engine (prec as above; tile_cfg 0..6 selects the workgroup tile).
  * For engine verification.  Returns 0 on success. */

@@ -148,6 +148,9 @@ _SIGS = {
     "librpa_amd_get_chi0s_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "librpa_amd_get_chi0q_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "librpa_amd_get_tfgrids": (C.c_int, [_hp, _op, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "librpa_amd_get_wc_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
+    "librpa_amd_get_sigma_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp]),
+    "librpa_amd_get_sigc_ks": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "librpa_amd_profile_json": (C.c_int, [C.c_char_p, C.c_int]),
     "librpa_amd_profile_reset": (None, []),
     "librpa_amd_get_stats": (C.c_int, [_hp, _dp, C.c_int]),
@@ -352,6 +355,26 @@ class Handler:
         self._call("librpa_amd_get_chi0q_block", ifreq, iq, I, J, re.ctypes.data_as(_dp), im.ctypes.data_as(_dp))
         return re + 1j * im
 
+    def get_wc_block(self, itau, I, J, R, naI, naJ):
+        out = np.zeros((naI, naJ))
+        r = (C.c_int * 3)(*[int(x) for x in R])
+        self._call("librpa_amd_get_wc_block", itau, I, J, r, out.ctypes.data_as(_dp))
+        return out
+
+    def get_sigma_block(self, ispin, ifreq, I, J, R, nbI, nbJ):
+        re = np.zeros((nbI, nbJ))
+        im = np.zeros((nbI, nbJ))
+        r = (C.c_int * 3)(*[int(x) for x in R])
+        self._call("librpa_amd_get_sigma_block", ispin, ifreq, I, J, r, re.ctypes.data_as(_dp),
+                   im.ctypes.data_as(_dp))
+        return re + 1j * im
+
+    def get_sigc_ks(self, ispin, ik, state, nfreq):
+        re = np.zeros(nfreq)
+        im = np.zeros(nfreq)
+        self._call("librpa_amd_get_sigc_ks", ispin, ik, state, re.ctypes.data_as(_dp), im.ctypes.data_as(_dp))
+        return re + 1j * im
+
     def get_tfgrids(self, opts: Options):
         n = opts.nfreq
         arrs = [np.zeros(n) for _ in range(4)] + [np.zeros((n, n)) for _ in range(3)]
@@ -367,8 +390,10 @@ class Handler:
         return dict(zip(keys, out.tolist()))
 
 
-def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True):
-    """Push a librpa_amd.synth.System through the C API (mirrors the driver's read_* calls)."""
+def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut: bool = False):
+    """Push a librpa_amd.synth.System through the C API (mirrors the driver's read_* calls).
+
+    ``vq_cut`` also passes the Coulomb matrix as the truncated one used for W (G0W0)."""
     ns, nk, nb, nbas = sys.nspin, sys.nk, sys.nbands, sys.nbasis
     h.set_scf_dimension(ns, nk, nb, nbas)
     h.set_wg_ekb_efermi(ns, nk, nb, sys.occ.reshape(-1), sys.eps.reshape(-1), sys.efermi)
@@ -391,3 +416,5 @@ def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True):
                 for J in range(I, sys.natom):
                     b = V[off[I]:off[I + 1], off[J]:off[J + 1]]
                     h.set_aux_bare_coulomb_k_atom_pair(ik, I, J, b.shape[0], b.shape[1], b.real, b.imag, 0.0)
+                    if vq_cut:
+                        h.set_aux_cut_coulomb_k_atom_pair(ik, I, J, b.shape[0], b.shape[1], b.real, b.imag, 0.0)

@@ -64,6 +64,7 @@ void set_wfc_common(LibrpaHandler *h, int ispin, int ik, int nstates, int nbasis
     w.resize((size_t)nstates * nbasis);
     for (size_t i = 0; i < w.size(); ++i) w[i] = ri ? cplx(ri[2 * i], ri[2 * i + 1]) : cplx(re[i], im[i]);
     d.rpa.reset();
+    d.gw.reset();
 }
 
 void set_vq_common(Dataset &d, std::map<std::tuple<int, int, int>, std::vector<cplx>> &store, int ik, int I, int J,
@@ -76,6 +77,7 @@ void set_vq_common(Dataset &d, std::map<std::tuple<int, int, int>, std::vector<c
     }
     if (mx >= thr) store[{ik, I, J}] = std::move(b);  // input.cpp:580-583
     d.rpa.reset();
+    d.gw.reset();
 }
 
 void set_vq_2d(Dataset &d, std::map<std::tuple<int, int, int>, std::vector<cplx>> &store, int ik, int mu0, int mu1,
@@ -97,6 +99,7 @@ void set_vq_2d(Dataset &d, std::map<std::tuple<int, int, int>, std::vector<cplx>
             blk[(size_t)(g - off[I]) * d.nb_aux[J] + (h - off[J])] = cplx(re[idx], im[idx]);
         }
     d.rpa.reset();
+    d.gw.reset();
 }
 
 void prepare_grid(Dataset &d, const LibrpaOptions &o) {
@@ -122,6 +125,10 @@ std::string json_escape(const std::string &s) {
 }
 
 }  // namespace
+
+namespace lra {
+void lra_prepare_grid(Dataset &d, const LibrpaOptions &o) { prepare_grid(d, o); }
+}  // namespace lra
 
 extern "C" {
 
@@ -246,6 +253,7 @@ void librpa_set_scf_dimension(LibrpaHandler *h, int nspins, int nkpts, int nstat
         mf.wfc.assign((size_t)nspins * nkpts, {});
         mf.dims_set = true;
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -261,6 +269,7 @@ void librpa_set_wg_ekb_efermi(LibrpaHandler *h, int nspins, int nkpts, int nstat
         mf.efermi = efermi;
         mf.eig_set = true;
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -288,6 +297,7 @@ void librpa_set_ao_basis_wfc(LibrpaHandler *h, int natoms, const size_t *nbs) {
         auto &d = ds_of(h);
         d.nb_wfc.assign(nbs, nbs + natoms);
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -296,6 +306,7 @@ void librpa_set_ao_basis_aux(LibrpaHandler *h, int natoms, const size_t *nbs) {
         auto &d = ds_of(h);
         d.nb_aux.assign(nbs, nbs + natoms);
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -313,6 +324,25 @@ void librpa_set_atoms(LibrpaHandler *h, int natoms, const int *types, const doub
         auto &d = ds_of(h);
         d.types.assign(types, types + natoms);
         d.pos.assign(posi_cart, posi_cart + 3 * (size_t)natoms);
+        // fractional coordinates only when the lattice is already known (input.cpp:344-347,
+        // geometry.cpp:108-126): frac = cart * latvec^{-1}
+        d.frac.clear();
+        if (d.lat_set) {
+            const double *a = d.latvec;
+            const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                               a[2] * (a[3] * a[7] - a[4] * a[6]);
+            if (std::fabs(det) < 1e-300) LRA_THROW("singular lattice vectors");
+            const double inv[9] = {(a[4] * a[8] - a[5] * a[7]) / det, (a[2] * a[7] - a[1] * a[8]) / det,
+                                   (a[1] * a[5] - a[2] * a[4]) / det, (a[5] * a[6] - a[3] * a[8]) / det,
+                                   (a[0] * a[8] - a[2] * a[6]) / det, (a[2] * a[3] - a[0] * a[5]) / det,
+                                   (a[3] * a[7] - a[4] * a[6]) / det, (a[1] * a[6] - a[0] * a[7]) / det,
+                                   (a[0] * a[4] - a[1] * a[3]) / det};
+            d.frac.assign(3 * (size_t)natoms, 0.0);
+            for (int i = 0; i < natoms; ++i)
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < 3; ++r) d.frac[3 * (size_t)i + c] += posi_cart[3 * i + r] * inv[3 * r + c];
+        }
+        d.gw.reset();
     });
 }
 
@@ -335,6 +365,7 @@ void librpa_set_kgrids_kvec(LibrpaHandler *h, int nk1, int nk2, int nk3, const d
         d.ibz_map.resize(n);
         for (int k = 0; k < n; ++k) d.ibz_map[k] = k;
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -357,6 +388,7 @@ void librpa_set_lri_coeff(LibrpaHandler *h, LibrpaParallelRouting, int I, int J,
         const size_t n = (size_t)nbasis_i * nbasis_j * naux_mu;
         d.cs[{I, J, R[0], R[1], R[2]}] = std::vector<double>(Cs_in, Cs_in + n);
         d.rpa.reset();
+        d.gw.reset();
     });
 }
 
@@ -463,6 +495,7 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             auto eng = d.rpa;
             if (!eng || eng->prec != d.chi0_prec || eng->planned_ntau != d.grid.n) {
                 d.rpa.reset();
+                d.gw.reset();
                 eng = std::make_shared<RpaEngine>();
                 eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
             }
@@ -558,6 +591,54 @@ int librpa_amd_get_chi0s_block(LibrpaHandler *h, int itau, int I, int J, const i
             return 0;
         },
         1);
+}
+
+int librpa_amd_get_wc_block(LibrpaHandler *h, int itau, int I, int J, const int R[3], double *out) {
+    return guarded(
+        [&]() -> int {
+            auto &d = ds_of(h);
+            if (!d.gw) LRA_THROW("no G0W0 data: call librpa_build_g0w0_sigma first");
+            auto &g = *d.gw;
+            if (itau < 0 || itau >= g.ntau || I < 0 || J < 0 || I >= g.natom || J >= g.natom)
+                LRA_THROW("index out of range");
+            g.w_block(itau, I, J, g.rpa->fold_index(R[0], R[1], R[2]), out);
+            return 0;
+        },
+        1);
+}
+
+int librpa_amd_get_sigma_block(LibrpaHandler *h, int ispin, int ifreq, int I, int J, const int R[3], double *re,
+                               double *im) {
+    return guarded(
+        [&]() -> int {
+            auto &d = ds_of(h);
+            if (!d.gw) LRA_THROW("no G0W0 data: call librpa_build_g0w0_sigma first");
+            auto &g = *d.gw;
+            if (ispin < 0 || ispin >= g.nspin || ifreq < 0 || ifreq >= g.ntau || I < 0 || J < 0 || I >= g.natom ||
+                J >= g.natom)
+                LRA_THROW("index out of range");
+            g.sigma_block(ispin, ifreq, I, J, g.rpa->fold_index(R[0], R[1], R[2]), re, im);
+            return 0;
+        },
+        1);
+}
+
+int librpa_amd_get_sigc_ks(LibrpaHandler *h, int ispin, int ik, int i_state, double *re, double *im) {
+    return guarded(
+        [&]() -> int {
+            auto &d = ds_of(h);
+            if (!d.gw) LRA_THROW("no G0W0 data: call librpa_build_g0w0_sigma first");
+            auto &g = *d.gw;
+            if (ispin < 0 || ispin >= g.nspin || ik < 0 || ik >= g.nk || i_state < 0 || i_state >= d.mf.nbands)
+                LRA_THROW("index out of range");
+            auto v = g.ks_diag(d, ispin, ik, {i_state});
+            for (size_t w = 0; w < v[0].size(); ++w) {
+                re[w] = v[0][w].real();
+                im[w] = v[0][w].imag();
+            }
+            return (int)v[0].size();
+        },
+        -1);
 }
 
 int librpa_amd_get_chi0q_block(LibrpaHandler *h, int ifreq, int iq, int I, int J, double *re, double *im) {

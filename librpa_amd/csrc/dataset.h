@@ -53,6 +53,7 @@ struct Dataset {
     double latvec[9] = {0}, Gvec[9] = {0};
     std::vector<int> types;
     std::vector<double> pos;
+    std::vector<double> frac;  // fractional coordinates, empty unless the lattice was set first
     int period[3] = {0, 0, 0};
     std::vector<double> kcart, kfrac;  // [k][3]
     std::vector<int> ibz_map;

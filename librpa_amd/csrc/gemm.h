@@ -8,7 +8,11 @@
 //   * row indirection: row m of A and C may come from a table of absolute
 //     element offsets (gathered rows, scattered outputs);
 //   * column split on C: output column c goes to (c / ncb) * cbs + c % ncb,
-//     which writes a [j][rows][nu] layout directly from a [rows][(j, nu)] GEMM.
+//     which writes a [j][rows][nu] layout directly from a [rows][(j, nu)] GEMM;
+//   * column table on C: the within-block part c % ncb may be replaced by an
+//     entry of a table of element offsets (scattered output columns);
+//   * atomic epilogue: C += alpha A B with atomic adds, for outputs that several
+//     problems of one launch accumulate into.
 #pragma once
 
 #include <vector>
@@ -33,6 +37,7 @@ struct GemmProb {
     int lda, ldb, ldc;
     int rt_off;    // offset into the row table, -1 if not row-indirect
     int kmode;     // triangular operand: 0 none, 1 B lower (k < n is zero), 2 A = L^T (k < m is zero)
+    int ct_off;    // offset into the column table, -1 if none
 };
 
 // Tile configurations (BM x BN, 256 threads).
@@ -54,13 +59,15 @@ const char *tile_name(int cfg);
 struct GemmBatch {
     std::vector<GemmProb> probs;
     std::vector<long long> rowtab;  // pairs (a_row_off, c_row_off)
+    std::vector<long long> coltab;  // C column offsets (within a column-split block)
     int cfg = T64x64;
     bool trans_a = false;  // A stored as A^T (element (m,k) at a_off + k*lda + m)
     bool trans_b = false;  // B stored as B^T (element (k,n) at b_off + n*ldb + k)
     bool lower_only = false;
+    bool atomic = false;   // C += alpha A B by atomic adds (beta must be 1)
     int total_tiles = 0;
     double flops = 0;      // algorithmic flops of one launch (2 M N K summed)
-    DevBuf d_probs, d_rowtab;
+    DevBuf d_probs, d_rowtab, d_coltab;
 
     void add(const GemmProb &p) { probs.push_back(p); }
     void finalize(int cfg_, hipStream_t s = 0);

@@ -72,8 +72,9 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
                                                     const typename ModeTraits<MODE>::T *__restrict__ B,
                                                     typename ModeTraits<MODE>::T *__restrict__ C,
                                                     const long long *__restrict__ rowtab,
+                                                    const long long *__restrict__ coltab,
                                                     typename ModeTraits<MODE>::T alpha,
-                                                    typename ModeTraits<MODE>::T beta, int lower_only) {
+                                                    typename ModeTraits<MODE>::T beta, int lower_only, int atomic) {
     using T = typename ModeTraits<MODE>::T;
     using V = typename ModeTraits<MODE>::V;
     constexpr int BK = ModeTraits<MODE>::BK;
@@ -234,6 +235,7 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
 
     // epilogue: acc[a][b][r] holds C[row][n0 + wn*WTN + b*16 + lane%16]
     const bool split = P.ncb < P.N;
+    const bool ct = P.ct_off >= 0;
 #pragma unroll
     for (int a = 0; a < FM; ++a) {
 #pragma unroll
@@ -247,11 +249,18 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
             for (int b = 0; b < FN; ++b) {
                 const int gn = n0 + wn * WTN + b * 16 + (lane & 15);
                 if (gn >= P.N) continue;
-                const long long col = split ? (long long)(gn / P.ncb) * P.cbs + gn % P.ncb : gn;
-                T* dst = C + crow + col;
-                T v = alpha * acc[a][b][r];
-                if (beta != T(0)) v += beta * *dst;
-                *dst = v;
+                const int inner = split ? gn % P.ncb : gn;
+                long long col = ct ? coltab[P.ct_off + inner] : inner;
+                if (split) col += (long long)(gn / P.ncb) * P.cbs;
+                T *dst = C + crow + col;
+                const T v0 = alpha * acc[a][b][r];
+                if (atomic) {
+                    atomicAdd(dst, v0);
+                } else {
+                    T v = v0;
+                    if (beta != T(0)) v += beta * *dst;
+                    *dst = v;
+                }
             }
         }
     }
@@ -275,6 +284,7 @@ GemmProb make_prob(int M, int N, int K, long long a_off, int lda, long long b_of
     p.ncb = 1 << 30; p.cbs = 0;
     p.a_off = a_off; p.lda = lda; p.b_off = b_off; p.ldb = ldb; p.c_off = c_off; p.ldc = ldc;
     p.rt_off = -1;
+    p.ct_off = -1;
     return p;
 }
 
@@ -292,6 +302,7 @@ void GemmBatch::finalize(int cfg_, hipStream_t s) {
     total_tiles = t0;
     d_probs = upload(probs, s);
     if (!rowtab.empty()) d_rowtab = upload(rowtab, s);
+    if (!coltab.empty()) d_coltab = upload(coltab, s);
 }
 
 template <int MODE, int CFG, bool TA, bool TB>
@@ -304,8 +315,8 @@ static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C
     constexpr int WN = 4 / WM;
     hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB>), dim3(b.total_tiles), dim3(256), 0, s,
                        b.d_probs.as<GemmProb>(), (int)b.probs.size(), b.total_tiles, static_cast<const T *>(A),
-                       static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(), (T)alpha,
-                       (T)beta, (int)b.lower_only);
+                       static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(),
+                       b.d_coltab.as<long long>(), (T)alpha, (T)beta, (int)b.lower_only, (int)b.atomic);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -337,6 +348,7 @@ void gemm_run(Prec prec, const GemmBatch &b, const void *A, const void *B, void 
               hipStream_t s) {
     if (b.total_tiles == 0) return;
     if (b.trans_a && !b.rowtab.empty()) LRA_THROW("row indirection needs non-transposed A");
+    if (b.atomic && beta != 1.0) LRA_THROW("atomic GEMM epilogue accumulates (beta must be 1)");
     switch (prec) {
         case Prec::F64: launch_mode<0>(b, A, B, C, alpha, beta, s); break;
         case Prec::F32: launch_mode<1>(b, A, B, C, alpha, beta, s); break;

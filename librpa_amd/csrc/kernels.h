@@ -62,8 +62,11 @@ struct CholWork {
 void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w,
                      hipStream_t s);
 // The same for nbatch matrices at A + b * strideA, all factorised in the same launches;
-// matrix b adds its ln|det| to logdet[b * ld_stride].
+// matrix b adds its ln|det| to logdet[b * ld_stride].  With ncols in (0, n) only the
+// first ncols columns are eliminated: the lower triangle of the trailing block then
+// holds its Schur complement A22 - A21 A11^{-1} A21^T and logdet gets ln|det A11|.
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
-                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s);
+                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s,
+                             int ncols = -1);
 
 }  // namespace lra

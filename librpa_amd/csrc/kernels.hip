@@ -385,24 +385,25 @@ struct CholPlan {
     std::vector<GemmBatch> trsm, syrk;
 };
 
-static std::map<std::tuple<int, int, int, long long, hipStream_t>, CholPlan> &chol_plans() {
-    static std::map<std::tuple<int, int, int, long long, hipStream_t>, CholPlan> p;
+static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan> &chol_plans() {
+    static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan> p;
     return p;
 }
 
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
-                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
+                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols) {
+    const int nf = (ncols > 0 && ncols < n) ? ncols : n;
     const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
     const long long pstride = 2LL * n * CHOL_NB;
     w.linv.reserve(sizeof(double) * wstride * nbatch);
     w.panel.reserve(sizeof(double) * pstride * nbatch);
-    auto key = std::make_tuple(n, lda, nbatch, strideA, s);
+    auto key = std::make_tuple(n, lda, nbatch, strideA, s, nf);
     auto &plans = chol_plans();
     auto it = plans.find(key);
     if (it == plans.end()) {
         CholPlan cp;
-        for (int j0 = 0; j0 < n; j0 += CHOL_NB) {
-            const int nb = std::min(CHOL_NB, n - j0);
+        for (int j0 = 0; j0 < nf; j0 += CHOL_NB) {
+            const int nb = std::min(CHOL_NB, nf - j0);
             const int m = n - j0 - nb;
             cp.j0s.push_back(j0);
             cp.nbs.push_back(nb);

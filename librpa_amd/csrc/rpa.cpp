@@ -398,11 +398,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
-    const auto &mf = ds.mf;
-    auto &prof = profiler();
-    const double t0 = now_s();
-    const int ntau = g.n;
+void RpaEngine::prepare_gf(const MeanField &mf, int nbands_g) {
     // band ranges with non-zero weight for each sign
     for (int sg = 0; sg < 2; ++sg) {
         int lo = nbands, hi = 0;
@@ -428,36 +424,48 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                                 (long long)k * 2 * NAO * NAO, NAO));
         b.finalize(T128x128, stream);
     }
+    gf_nbands = nbands_g;
+}
+
+void RpaEngine::build_gf(const MeanField &mf, double tau, int s) {
+    auto &prof = profiler();
+    const size_t zz_s = (size_t)nk * 2 * nbands * 2 * NAO * s;
+    const size_t w_s = (size_t)nk * nbands * NAO * s;
+    const size_t e_s = (size_t)nk * nbands * s;
     const long long gpm_bytes = (long long)nR * NAO * 2 * NAO * ts;
+    prof.begin("gf_build", stream);
+    for (int sg = 0; sg < 2; ++sg) {
+        const double t = sg == 0 ? tau : -tau;
+        if (band_hi[sg] > band_lo[sg]) {
+            launch_scale_bands(d_wre.as<double>() + w_s, d_wim.as<double>() + w_s, d_eps.as<double>() + e_s,
+                               d_wg.as<double>() + e_s, d_Zs.as<double>(), nk, nbands, NAO, band_lo[sg], band_hi[sg],
+                               t, mf.efermi, nspin, gf_nbands, stream);
+            gemm_run(Prec::F64, gb[sg], d_ZZ.as<double>() + zz_s, d_Zs.get(), d_Mst.get(), 1.0, 0.0, stream);
+        } else {
+            HIP_CHECK(hipMemsetAsync(d_Mst.get(), 0, d_Mst.bytes(), stream));
+        }
+        const double sgn = t < 0 ? -1.0 : 1.0;
+        if (prec == Prec::F64)
+            launch_k2r<double>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
+                               d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<double>(), nk, nR, NAO, sgn, stream);
+        else
+            launch_k2r<float>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
+                              d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<float>(), nk, nR, NAO, sgn, stream);
+    }
+    prof.end("gf_build", stream, gb[0].flops + gb[1].flops, (double)gpm_bytes);
+}
+
+void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
+    const auto &mf = ds.mf;
+    auto &prof = profiler();
+    const double t0 = now_s();
+    const int ntau = g.n;
+    prepare_gf(mf, nbands_g);
     HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
         for (int s = 0; s < nspin; ++s) {
-            const size_t zz_s = (size_t)nk * 2 * nbands * 2 * NAO * s;
-            const size_t w_s = (size_t)nk * nbands * NAO * s;
-            const size_t e_s = (size_t)nk * nbands * s;
-            prof.begin("gf_build", stream);
-            for (int sg = 0; sg < 2; ++sg) {
-                const double t = sg == 0 ? tau : -tau;
-                if (band_hi[sg] > band_lo[sg]) {
-                    launch_scale_bands(d_wre.as<double>() + w_s, d_wim.as<double>() + w_s, d_eps.as<double>() + e_s,
-                                       d_wg.as<double>() + e_s, d_Zs.as<double>(), nk, nbands, NAO, band_lo[sg],
-                                       band_hi[sg], t, mf.efermi, nspin, nbands_g, stream);
-                    gemm_run(Prec::F64, gb[sg], d_ZZ.as<double>() + zz_s, d_Zs.get(), d_Mst.get(), 1.0, 0.0, stream);
-                } else {
-                    HIP_CHECK(hipMemsetAsync(d_Mst.get(), 0, d_Mst.bytes(), stream));
-                }
-                const double sgn = t < 0 ? -1.0 : 1.0;
-                if (prec == Prec::F64)
-                    launch_k2r<double>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
-                                       d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<double>(), nk, nR, NAO, sgn,
-                                       stream);
-                else
-                    launch_k2r<float>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
-                                      d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<float>(), nk, nR, NAO, sgn,
-                                      stream);
-            }
-            prof.end("gf_build", stream, gb[0].flops + gb[1].flops, (double)gpm_bytes);
+            build_gf(mf, tau, s);
             char *chi_base = d_chi0s.as<char>() + (size_t)it * E_tot * ts;
             for (auto &c : chunks) {
                 prof.begin("chi0_gather", stream);

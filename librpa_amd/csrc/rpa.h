@@ -50,6 +50,7 @@ struct RpaEngine {
 
     // G build
     int band_lo[2] = {0, 0}, band_hi[2] = {0, 0};  // per sign (0: +tau, 1: -tau)
+    int gf_nbands = -1;
     GemmBatch gb[2];
 
     // ---- device data ----
@@ -72,6 +73,9 @@ struct RpaEngine {
     int planned_ntau = 0;
 
     void plan(const Dataset &ds, Prec p, Prec pe, int ntau);
+    // G(R, +-tau) of spin s into d_Gpm (prepare_gf once per band cut first)
+    void prepare_gf(const MeanField &mf, int nbands_g);
+    void build_gf(const MeanField &mf, double tau, int s);
     void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);
     RpaResult ec(const Dataset &ds, const TFGrid &g);
     // dense chi0(q, omega) block for verification

@@ -129,7 +129,10 @@ def _cs_pairs(latvec, pos, cutoff=CS_CUTOFF):
 
 
 def build_system(name: str, latvec, positions, types, species: List[Species], period, nfreq,
-                 seed: int, with_vq: bool = True, vq_device: Optional[str] = None) -> System:
+                 seed: int, with_vq: bool = True, vq_device: Optional[str] = None,
+                 real_gamma_v: bool = False) -> System:
+    """``real_gamma_v`` keeps only Re v(k) at k = 0 (a time-reversal-symmetric crystal has a real
+    Coulomb matrix there; the G0W0 path needs it)."""
     latvec = np.asarray(latvec, dtype=np.float64)
     positions = np.asarray(positions, dtype=np.float64)
     types = np.asarray(types, dtype=np.int64)
@@ -180,6 +183,8 @@ def build_system(name: str, latvec, positions, types, species: List[Species], pe
         for ik in range(nk):
             b = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * scale[None, :]
             vq[ik] = _bbh(b, vq_device) / n + 1e-3 * np.eye(n)
+            if real_gamma_v and np.all(kfrac[ik] == 0):
+                vq[ik] = vq[ik].real.astype(np.complex128)
     return System(name=name, latvec=latvec, positions=positions, types=types, nao=nao, nabf=nabf,
                   period=tuple(period), kfrac=kfrac, eps=eps, occ=occ, efermi=float(efermi), wfc=wfc,
                   cs=cs, vq=vq, nfreq=nfreq)
@@ -253,23 +258,24 @@ def slab256(seed=3, with_vq=True, nfreq=16, vq_device=None):
 
 
 def cubic64(seed=4, with_vq=True, nfreq=24, vq_device=None):
-    """4x4x4 jittered lattice, spacing 4.5 bohr, species 50/50, 2x2x2 k (G0W0 workload)."""
+    """4x4x4 jittered lattice, spacing 4.5 bohr, species 50/50, 2x2x2 k (G0W0 workload; v(0) real)."""
     rng = np.random.default_rng(seed + 1000)
     pos = _jittered(4, 4, 4, 4.5, 0.3, rng)
     types = rng.permutation(np.array([0] * 32 + [1] * 32))
     return build_system("cubic64", np.diag([18.0, 18.0, 18.0]), pos, types, [SPEC_A, SPEC_B],
-                        (2, 2, 2), nfreq, seed, with_vq, vq_device)
+                        (2, 2, 2), nfreq, seed, with_vq, vq_device, real_gamma_v=True)
 
 
 def tiny(seed=7, natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=6.0,
-         nfreq=6, with_vq=True):
+         nfreq=6, with_vq=True, real_gamma_v=False):
     """A few-function system for brute-force checks (mixed species, k points)."""
     species = [Species(nao[i % len(nao)], nabf[i % len(nabf)], nocc[i % len(nocc)])
                for i in range(natom)]
     rng = np.random.default_rng(seed + 1000)
     pos = rng.uniform(0, a, (natom, 3))
     lat = np.eye(3) * a
-    return build_system("tiny", lat, pos, list(range(natom)), species, period, nfreq, seed, with_vq)
+    return build_system("tiny", lat, pos, list(range(natom)), species, period, nfreq, seed, with_vq,
+                        real_gamma_v=real_gamma_v)
 
 
 WORKLOADS = {

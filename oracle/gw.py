@@ -225,38 +225,42 @@ def sigma_factorized(sys, cs, G, Wt):
             acc = np.zeros((n, int(sys.nao[J]), int(sys.nabf[J])))
             for (L, Rb) in nb.get(J, []):
                 g = G[fold_R(np.array(S) + np.array(Rb), per)][:, ao.sl(L)]
-                acc += np.einsum('xl,njl->xjn', g, cs[(J, L, Rb)])
+                acc += np.tensordot(g, cs[(J, L, Rb)], axes=([1], [2])).transpose(0, 2, 1)
             U[(J, S)] = acc
     main = {R: np.zeros((n, n)) for R in Rs}
     bterm = {R: np.zeros((n, n)) for R in Rs}
     for I in range(sys.natom):
+        ni, nmu = int(sys.nao[I]), int(sys.nabf[I])
         Y = {}
         for J in range(sys.natom):
+            nj, nnu = int(sys.nao[J]), int(sys.nabf[J])
             for R in Rs:
                 Rv = np.array(R)
                 w = _wblk(Wt, abf, I, J, Rv, per)
                 # a2b2
-                acc = np.zeros((int(sys.nao[I]), int(sys.nao[J])))
+                acc = np.zeros((ni, nj))
                 for (K, Ra) in nb.get(I, []):
-                    u = U[(J, fold_R(Rv - np.array(Ra), per))][ao.sl(K)]        # (k, j, nu)
-                    t = np.einsum('mn,kjn->mkj', w, u)
-                    acc += np.einsum('mik,mkj->ij', cs[(I, K, Ra)], t)
+                    u = U[(J, fold_R(Rv - np.array(Ra), per))][ao.sl(K)]          # (k, j, nu)
+                    t = np.tensordot(w, u, axes=([1], [2]))                      # (mu, k, j)
+                    c = cs[(I, K, Ra)]                                           # (mu, i, k)
+                    acc += np.tensordot(c, t, axes=([0, 2], [0, 1]))             # (i, j)
                 main[fold_R(Rv, per)][ao.sl(I), ao.sl(J)] += acc
                 # a1b2
-                P = np.einsum('mn,ijn->mij', w, U[(J, fold_R(Rv, per))][ao.sl(I)])
+                P = np.tensordot(w, U[(J, fold_R(Rv, per))][ao.sl(I)], axes=([1], [2]))   # (mu, i, j)
                 for (K, Ra) in nb.get(I, []):
-                    bterm[fold_R(Rv - np.array(Ra), per)][ao.sl(K), ao.sl(J)] += np.einsum(
-                        'mik,mij->kj', cs[(I, K, Ra)], P)
-                # a1b1 intermediate
+                    bterm[fold_R(Rv - np.array(Ra), per)][ao.sl(K), ao.sl(J)] += np.tensordot(
+                        cs[(I, K, Ra)], P, axes=([0, 1], [0, 1]))
+                # a1b1 intermediate: y(mu, i, l) = sum_{nu, j} w(mu, nu) g(i, j) c(nu, j, l)
                 g = _gblk(G, ao, I, J, Rv, per)
                 for (L, Rb) in nb.get(J, []):
                     S = fold_R(Rv + np.array(Rb), per)
-                    y = np.einsum('mn,ij,njl->mil', w, g, cs[(J, L, Rb)])
+                    wc = np.tensordot(w, cs[(J, L, Rb)], axes=([1], [0]))        # (mu, j, l)
+                    y = np.einsum('ij,mjl->mil', g, wc, optimize=True)
                     Y[(L, S)] = Y.get((L, S), 0) + y
         for (K, Ra) in nb.get(I, []):
+            c2 = cs[(I, K, Ra)].reshape(nmu * ni, -1)                            # ((mu, i), k)
             for (L, S), y in Y.items():
-                main[fold_R(np.array(S) - np.array(Ra), per)][ao.sl(K), ao.sl(L)] += np.einsum(
-                    'mik,mil->kl', cs[(I, K, Ra)], y)
+                main[fold_R(np.array(S) - np.array(Ra), per)][ao.sl(K), ao.sl(L)] += c2.T @ y.reshape(nmu * ni, -1)
     out = {}
     for R in Rs:
         mR = fold_R(-np.array(R), per)
