@@ -25,6 +25,11 @@ void librpa_init_global(LibrpaSwitch switch_redirect_stdout, const char *redirec
                         LibrpaSwitch switch_process_output);
 #endif
 
+/* Entry point of the Fortran binding (binding/fortran/librpa_f03.f90:316, implemented in
+ * src/api/global.cpp:56-61). */
+void librpa_init_global_fortran(int *f_comm, LibrpaSwitch switch_redirect_stdout, const char *redirect_path,
+                                LibrpaSwitch switch_process_output);
+
 /* Release every instance (librpa_global.h:72). */
 void librpa_finalize_global(void);
 

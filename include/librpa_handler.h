@@ -29,6 +29,10 @@ LibrpaHandler *librpa_create_handler(MPI_Comm comm);
 /* Free an instance and all its device data; NULL is ignored (librpa_handler.h:57). */
 void librpa_destroy_handler(LibrpaHandler *h);
 
+/* Entry point of the Fortran binding (binding/fortran/librpa_f03.f90:407, implemented in
+ * src/api/handler.cpp:26-30): Fortran communicator handle by reference. */
+LibrpaHandler *librpa_create_handler_fortran(int *f_comm);
+
 #ifdef __cplusplus
 }
 #endif

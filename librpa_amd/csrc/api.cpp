@@ -149,6 +149,8 @@ void librpa_init_global(LibrpaSwitch, const char *, LibrpaSwitch) {
     });
 }
 
+void librpa_init_global_fortran(int *, LibrpaSwitch a, const char *p, LibrpaSwitch b) { librpa_init_global(a, p, b); }
+
 void librpa_finalize_global(void) {
     guarded_void([&]() {
         g_instances.assign(1, nullptr);  // instance_manager.cpp:65-70
@@ -185,6 +187,10 @@ LibrpaHandler *librpa_create_handler(MPI_Comm) {
             return h;
         },
         (LibrpaHandler *)nullptr);
+}
+
+LibrpaHandler *librpa_create_handler_fortran(int *f_comm) {
+    return librpa_create_handler((MPI_Comm)(f_comm ? *f_comm : 0));
 }
 
 void librpa_destroy_handler(LibrpaHandler *h) {
