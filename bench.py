@@ -61,7 +61,7 @@ def run_steps(L, h, o, nk, prec, steps, warmup):
 
 def dominant_kernel(prof, prec_name):
     """Largest-time region among the single-kernel GEMM regions."""
-    cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_TB", "chi0_contract",
+    cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_X", "chi0_contract",
                                                             "ec_gemm"))}
     name, (ms, calls, flops, _bytes) = max(cands.items(), key=lambda kv: kv[1][0])
     ach = flops / (ms * 1e-3) / 1e12

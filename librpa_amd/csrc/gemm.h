@@ -12,7 +12,9 @@
 //   * column table on C: the within-block part c % ncb may be replaced by an
 //     entry of a table of element offsets (scattered output columns);
 //   * atomic epilogue: C += alpha A B with atomic adds, for outputs that several
-//     problems of one launch accumulate into.
+//     problems of one launch accumulate into;
+//   * K table: the inner index k may address A and B through a table of element
+//     offsets (a_k, b_k), for contractions whose summed index spans several strides.
 #pragma once
 
 #include <vector>
@@ -38,6 +40,7 @@ struct GemmProb {
     int rt_off;    // offset into the row table, -1 if not row-indirect
     int kmode;     // triangular operand: 0 none, 1 B lower (k < n is zero), 2 A = L^T (k < m is zero)
     int ct_off;    // offset into the column table, -1 if none
+    int kt_off;    // offset (in pairs) into the K table, -1 if none
 };
 
 // Tile configurations (BM x BN, 256 threads).
@@ -60,6 +63,7 @@ struct GemmBatch {
     std::vector<GemmProb> probs;
     std::vector<long long> rowtab;  // pairs (a_row_off, c_row_off)
     std::vector<long long> coltab;  // C column offsets (within a column-split block)
+    std::vector<long long> ktab;    // pairs (a_k, b_k): offsets of inner index k in A and B
     int cfg = T64x64;
     bool trans_a = false;  // A stored as A^T (element (m,k) at a_off + k*lda + m)
     bool trans_b = false;  // B stored as B^T (element (k,n) at b_off + n*ldb + k)
@@ -67,7 +71,7 @@ struct GemmBatch {
     bool atomic = false;   // C += alpha A B by atomic adds (beta must be 1)
     int total_tiles = 0;
     double flops = 0;      // algorithmic flops of one launch (2 M N K summed)
-    DevBuf d_probs, d_rowtab, d_coltab;
+    DevBuf d_probs, d_rowtab, d_coltab, d_ktab;
 
     void add(const GemmProb &p) { probs.push_back(p); }
     void finalize(int cfg_, hipStream_t s = 0);

@@ -66,13 +66,16 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
     return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
 }
 
-template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB>
+// VAR selects the addressing extras compiled in: 0 plain, 1 K table, 2 column table and
+// atomic epilogue (kept out of the plain kernels, where they cost registers).
+template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB, int VAR>
 __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
                                                     const typename ModeTraits<MODE>::T *__restrict__ A,
                                                     const typename ModeTraits<MODE>::T *__restrict__ B,
                                                     typename ModeTraits<MODE>::T *__restrict__ C,
                                                     const long long *__restrict__ rowtab,
                                                     const long long *__restrict__ coltab,
+                                                    const long long *__restrict__ ktab,
                                                     typename ModeTraits<MODE>::T alpha,
                                                     typename ModeTraits<MODE>::T beta, int lower_only, int atomic) {
     using T = typename ModeTraits<MODE>::T;
@@ -138,20 +141,27 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
             for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
 
     T ra[LA], rb[LB];
+    constexpr bool kt = VAR == 1;
     auto gload = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int gk = k0 + ak[i];
             T v = 0;
-            if (arow[i] >= 0 && gk < P.K) v = TA ? A[arow[i] + (long long)gk * P.lda] : A[arow[i] + gk];
+            if (arow[i] >= 0 && gk < P.K) {
+                const long long ko = kt ? ktab[2ll * (P.kt_off + gk)] : (TA ? (long long)gk * P.lda : (long long)gk);
+                v = A[arow[i] + ko];
+            }
             ra[i] = v;
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
             const int gk = k0 + bk[i], gn = n0 + bn[i];
             T v = 0;
-            if (gk < P.K && gn < P.N)
-                v = TB ? B[P.b_off + (long long)gn * P.ldb + gk] : B[P.b_off + (long long)gk * P.ldb + gn];
+            if (gk < P.K && gn < P.N) {
+                const long long ko =
+                    kt ? ktab[2ll * (P.kt_off + gk) + 1] : (TB ? (long long)gk : (long long)gk * P.ldb);
+                v = TB ? B[P.b_off + (long long)gn * P.ldb + ko] : B[P.b_off + ko + gn];
+            }
             rb[i] = v;
         }
     };
@@ -235,7 +245,7 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
 
     // epilogue: acc[a][b][r] holds C[row][n0 + wn*WTN + b*16 + lane%16]
     const bool split = P.ncb < P.N;
-    const bool ct = P.ct_off >= 0;
+    const bool ct = VAR == 2 && P.ct_off >= 0;
 #pragma unroll
     for (int a = 0; a < FM; ++a) {
 #pragma unroll
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
                 if (split) col += (long long)(gn / P.ncb) * P.cbs;
                 T *dst = C + crow + col;
                 const T v0 = alpha * acc[a][b][r];
-                if (atomic) {
+                if (VAR == 2 && atomic) {
                     atomicAdd(dst, v0);
                 } else {
                     T v = v0;
@@ -285,6 +295,7 @@ GemmProb make_prob(int M, int N, int K, long long a_off, int lda, long long b_of
     p.a_off = a_off; p.lda = lda; p.b_off = b_off; p.ldb = ldb; p.c_off = c_off; p.ldc = ldc;
     p.rt_off = -1;
     p.ct_off = -1;
+    p.kt_off = -1;
     return p;
 }
 
@@ -303,9 +314,10 @@ void GemmBatch::finalize(int cfg_, hipStream_t s) {
     d_probs = upload(probs, s);
     if (!rowtab.empty()) d_rowtab = upload(rowtab, s);
     if (!coltab.empty()) d_coltab = upload(coltab, s);
+    if (!ktab.empty()) d_ktab = upload(ktab, s);
 }
 
-template <int MODE, int CFG, bool TA, bool TB>
+template <int MODE, int CFG, bool TA, bool TB, int VAR>
 static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
                        hipStream_t s) {
     using T = typename ModeTraits<MODE>::T;
@@ -313,10 +325,11 @@ static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C
     constexpr int BN = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 128 : (CFG == 3) ? 64 : (CFG == 4) ? 128 : (CFG == 5) ? 256 : 64;
     constexpr int WM = (CFG == 2 || CFG == 5) ? 1 : 2;
     constexpr int WN = 4 / WM;
-    hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB>), dim3(b.total_tiles), dim3(256), 0, s,
+    hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB, VAR>), dim3(b.total_tiles), dim3(256), 0, s,
                        b.d_probs.as<GemmProb>(), (int)b.probs.size(), b.total_tiles, static_cast<const T *>(A),
                        static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(),
-                       b.d_coltab.as<long long>(), (T)alpha, (T)beta, (int)b.lower_only, (int)b.atomic);
+                       b.d_coltab.as<long long>(), b.d_ktab.as<long long>(), (T)alpha, (T)beta,
+                       (int)b.lower_only, (int)b.atomic);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -324,13 +337,13 @@ template <int MODE, bool TA, bool TB>
 static void launch_cfg(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
                        hipStream_t s) {
     switch (b.cfg) {
-        case 0: launch_one<MODE, 0, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 1: launch_one<MODE, 1, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 2: launch_one<MODE, 2, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 3: launch_one<MODE, 3, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 4: launch_one<MODE, 4, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 5: launch_one<MODE, 5, TA, TB>(b, A, B, C, alpha, beta, s); break;
-        case 6: launch_one<MODE, 6, TA, TB>(b, A, B, C, alpha, beta, s); break;
+        case 0: launch_one<MODE, 0, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 1: launch_one<MODE, 1, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 2: launch_one<MODE, 2, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 3: launch_one<MODE, 3, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 4: launch_one<MODE, 4, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 5: launch_one<MODE, 5, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 6: launch_one<MODE, 6, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
         default: LRA_THROW("bad tile cfg");
     }
 }
@@ -338,6 +351,20 @@ static void launch_cfg(const GemmBatch &b, const void *A, const void *B, void *C
 template <int MODE>
 static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
                         hipStream_t s) {
+    if (!b.ktab.empty()) {
+        // K-table batches: transposed A, plain B, 64x64 tiles
+        if (!b.trans_a || b.trans_b || b.cfg != T64x64 || b.atomic || !b.coltab.empty())
+            LRA_THROW("K-table GEMM: transposed A, plain B, 64x64 tiles, plain epilogue");
+        launch_one<MODE, 0, true, false, 1>(b, A, B, C, alpha, beta, s);
+        return;
+    }
+    if (b.atomic || !b.coltab.empty()) {
+        // column-table / atomic batches: plain A and B, 128x128 tiles
+        if (b.trans_a || b.trans_b || b.cfg != T128x128)
+            LRA_THROW("column-table GEMM: plain A and B, 128x128 tiles");
+        launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
+        return;
+    }
     if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);
     else if (b.trans_a) launch_cfg<MODE, true, false>(b, A, B, C, alpha, beta, s);
     else if (b.trans_b) launch_cfg<MODE, false, true>(b, A, B, C, alpha, beta, s);

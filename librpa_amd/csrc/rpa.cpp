@@ -159,10 +159,15 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         flops_chi0_alg = f * ntau;
     }
 
-    // ---- chunking over J by memory budget ----
+    // ---- W(K, J, S) of every J stays resident: [(s', j)][(K, k)][nu] per (J, S) ----
+    w_off.assign((size_t)natom * nR + 1, 0);
+    for (int J = 0; J < natom; ++J)
+        for (int S = 0; S < nR; ++S)
+            w_off[(size_t)J * nR + S + 1] = w_off[(size_t)J * nR + S] + 2LL * nb[J] * NAO * na[J];
+    // ---- chunking over J by memory budget (gathered G panels and T) ----
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const double budget = std::min(48e9, 0.35 * (double)free_b);
+    const double budget = std::min(48e9, 0.35 * (double)free_b - (double)w_off.back() * ts);
     chunks.clear();
     {
         int j = 0, o = 0;
@@ -170,21 +175,20 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
             Chunk c;
             c.j_begin = j;
             c.o_begin = o;
-            long long g = 0, w = 0, t = 0;
+            long long g = 0, t = 0;
             int jj = j, oo = o;
             while (jj < natom) {
                 long long g1 = (long long)nR * 2 * NAO * lsum[jj];
-                long long w1 = (long long)nR * 2 * nb[jj] * NAO * na[jj];
                 long long t1 = 0;
                 int o2 = oo;
                 while (o2 < (int)outs.size() && outs[o2].J == jj) { t1 += ksum[outs[o2].I] * na[jj]; ++o2; }
-                const double bytes = (double)(g + g1 + w + w1 + t + t1) * ts;
+                const double bytes = (double)(g + g1 + t + t1) * ts;
                 if (jj > j && bytes > budget) break;
-                g += g1; w += w1; t += t1; oo = o2; ++jj;
+                g += g1; t += t1; oo = o2; ++jj;
             }
             c.j_end = jj;
             c.o_end = oo;
-            c.ghat_elems = g; c.w_elems = w; c.t_elems = t;
+            c.ghat_elems = g; c.t_elems = t;
             chunks.push_back(std::move(c));
             j = jj;
             o = oo;
@@ -198,19 +202,18 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     for (int I = 0; I < natom; ++I)
         for (size_t n = 0; n < nbrs[I].size(); ++n) rev[nbrs[I][n].K].push_back({I, (int)n});
     std::set<int> classes(nb.begin(), nb.end());
+    std::map<std::tuple<int, int, int, int>, int> kt_pos;  // K-table start per species pair (per chunk)
 
     for (auto &c : chunks) {
+        kt_pos.clear();
         // offsets inside the chunk buffers
-        long long go = 0, wo = 0, to = 0;
+        long long go = 0, to = 0;
         for (int J = c.j_begin; J < c.j_end; ++J)
-            for (int S = 0; S < nR; ++S) {
+            for (int S = 0; S < nR; ++S)
                 for (int sg = 0; sg < 2; ++sg) { c.ghat_off.push_back(go); go += (long long)NAO * lsum[J]; }
-                c.w_off.push_back(wo);
-                wo += 2LL * nb[J] * NAO * na[J];
-            }
         for (int o = c.o_begin; o < c.o_end; ++o) { outs[o].toff = to; to += ksum[outs[o].I] * na[outs[o].J]; }
         auto ghat_at = [&](int J, int S, int sg) { return c.ghat_off[((size_t)(J - c.j_begin) * nR + S) * 2 + sg]; };
-        auto w_at = [&](int J, int S) { return c.w_off[(size_t)(J - c.j_begin) * nR + S]; };
+        auto w_at = [&](int J, int S) { return w_off[(size_t)J * nR + S]; };
 
         // gather jobs: Ghat(J,S,sg)[row][(n, l)] = G^sg(S + R2_n)[row][ao(L_n) + l]
         for (int J = c.j_begin; J < c.j_end; ++J)
@@ -242,27 +245,29 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                     pr.cbs = (long long)NAO * na[J];
                     if (pr.K > 0) c.wb.add(pr);
                 }
-        // T_B: per output, rows (n, k) from G_KJ(R - R1), B = W(I, J, R)
+        // G_KJ(R - R1) W(I, J, R) contracted with C^I directly: with
+        // X(I, J, R)[mu][(j, i)] = sum_(K,R1) C^I(K,R1)(mu, i, k) G_KJ(R - R1)(k, j) = W(J, -R)-block of I,
+        // chi0s(I, J, R) += sum_(s', j, i) W(I, -R)[(1 - s', i)][(J, j)][mu] W(J, R)[(s', j)][(I, i)][nu]
         for (int o = c.o_begin; o < c.o_end; ++o) {
             const auto &O = outs[o];
-            if (nbrs[O.I].empty()) continue;
-            GemmProb pr = make_prob(0, nb[O.I] * na[O.J], 2 * nb[O.J], 0, 0,
-                                    w_at(O.J, O.r) + ao[O.I] * na[O.J], NAO * na[O.J], 0, 0);
-            pr.rt_off = (int)(c.tb.rowtab.size() / 2);
-            int m = 0;
             const auto &Rv = cells[O.r];
-            for (size_t n = 0; n < nbrs[O.I].size(); ++n) {
-                const auto &nbk = nbrs[O.I][n];
-                const auto &R1 = cells[nbk.r];
-                const int rr = fold_index(Rv[0] - R1[0], Rv[1] - R1[1], Rv[2] - R1[2]);
-                for (int k = 0; k < nb[nbk.K]; ++k) {
-                    c.tb.rowtab.push_back(rr * gpm_r + (ao[nbk.K] + k) * 2LL * NAO + 2 * ao[O.J]);
-                    c.tb.rowtab.push_back(O.toff + (koff[O.I][n] + (long long)k * nb[O.I]) * na[O.J]);
-                    ++m;
-                }
+            const int mr = fold_index(-Rv[0], -Rv[1], -Rv[2]);
+            const int ni = nb[O.I], nj = nb[O.J];
+            const auto key = std::make_tuple(ni, na[O.I], nj, na[O.J]);
+            auto it = kt_pos.find(key);
+            if (it == kt_pos.end()) {
+                it = kt_pos.emplace(key, (int)(c.xb.ktab.size() / 2)).first;
+                for (int sp = 0; sp < 2; ++sp)
+                    for (int j = 0; j < nj; ++j)
+                        for (int i = 0; i < ni; ++i) {
+                            c.xb.ktab.push_back(((long long)(1 - sp) * ni + i) * NAO * na[O.I] + (long long)j * na[O.I]);
+                            c.xb.ktab.push_back(((long long)sp * nj + j) * NAO * na[O.J] + (long long)i * na[O.J]);
+                        }
             }
-            pr.M = m;
-            c.tb.add(pr);
+            GemmProb pr = make_prob(na[O.I], na[O.J], 2 * ni * nj, w_at(O.I, mr) + ao[O.J] * na[O.I], 0,
+                                    w_at(O.J, O.r) + ao[O.I] * na[O.J], 0, O.eoff, na[O.J]);
+            pr.kt_off = it->second;
+            c.xb.add(pr);
         }
         // T_A: per (J, S, K, class of I), rows (I, n, i) from G_IJ(S + R1), B = W(K, J, S)
         for (int J = c.j_begin; J < c.j_end; ++J)
@@ -305,22 +310,22 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         const bool bf = prec == Prec::BF16X3;
         (void)bf;
         c.wb.finalize(T128x128, stream);
-        c.tb.finalize(T128x64, stream);
+        c.xb.trans_a = true;
+        c.xb.finalize(T64x64, stream);
         c.ta.finalize(T64x128, stream);
         c.fin.finalize(T64x64, stream);
         c.d_jobs = upload(c.jobs, stream);
         c.d_colsrc = upload(c.colsrc, stream);
-        flops_chi0_exec += c.wb.flops + c.tb.flops + c.ta.flops + c.fin.flops;
+        flops_chi0_exec += c.wb.flops + c.xb.flops + c.ta.flops + c.fin.flops;
     }
     flops_chi0_exec *= ntau;
-    size_t max_g = 0, max_w = 0, max_t = 0;
+    size_t max_g = 0, max_t = 0;
     for (auto &c : chunks) {
         max_g = std::max(max_g, (size_t)c.ghat_elems);
-        max_w = std::max(max_w, (size_t)c.w_elems);
         max_t = std::max(max_t, (size_t)c.t_elems);
     }
     d_Ghat.alloc(max_g * ts);
-    d_W.alloc(max_w * ts);
+    d_W.alloc((size_t)w_off.back() * ts);
     d_T.alloc(max_t * ts);
 
     // ---- G build data ----
@@ -479,14 +484,16 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 prof.begin("chi0_W", stream);
                 gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
                 prof.end("chi0_W", stream, c.wb.flops);
-                prof.begin("chi0_TB", stream);
-                gemm_run(prec, c.tb, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                prof.end("chi0_TB", stream, c.tb.flops);
+            }
+            for (auto &c : chunks) {
+                prof.begin("chi0_X", stream);
+                gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                prof.end("chi0_X", stream, c.xb.flops);
                 prof.begin("chi0_TA", stream);
-                gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 1.0, stream);
+                gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                 prof.end("chi0_TA", stream, c.ta.flops);
                 prof.begin("chi0_contract", stream);
-                gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
                 prof.end("chi0_contract", stream, c.fin.flops);
             }
         }

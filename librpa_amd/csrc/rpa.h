@@ -38,15 +38,16 @@ struct RpaEngine {
     struct Chunk {
         int j_begin, j_end;          // atoms J in [j_begin, j_end)
         int o_begin, o_end;          // outputs
-        long long ghat_elems = 0, w_elems = 0, t_elems = 0;
-        std::vector<long long> ghat_off, w_off;  // per (J - j_begin, S, sigma) / (J - j_begin, S)
+        long long ghat_elems = 0, t_elems = 0;
+        std::vector<long long> ghat_off;  // per (J - j_begin, S, sigma)
         std::vector<GatherJob> jobs;
         std::vector<long long> colsrc;
         DevBuf d_jobs, d_colsrc;
         int max_gcol = 0;
-        GemmBatch wb, tb, ta, fin;
+        GemmBatch wb, xb, ta, fin;
     };
     std::vector<Chunk> chunks;
+    std::vector<long long> w_off;  // W(K, J, S) block of (J, S) in d_W
 
     // G build
     int band_lo[2] = {0, 0}, band_hi[2] = {0, 0};  // per sign (0: +tau, 1: -tau)
