@@ -54,6 +54,9 @@ enum TileCfg : int {
     T32x64 = 6,
     NUM_TILE_CFG = 7
 };
+// Largest square tile that runs well for a precision: 64x64 for FP64 (two waves per SIMD;
+// 128x128 FP64 tiles leave one wave per SIMD and measure ~20% slower), 128x128 otherwise.
+inline int big_tile(Prec p) { return p == Prec::F64 ? T64x64 : T128x128; }
 int tile_bm(int cfg);
 int tile_bn(int cfg);
 const char *tile_name(int cfg);

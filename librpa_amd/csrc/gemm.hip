@@ -27,9 +27,15 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 
+#ifndef LRA_GEMM_BK64
+#define LRA_GEMM_BK64 16
+#endif
+#ifndef LRA_GEMM_BK32
+#define LRA_GEMM_BK32 16
+#endif
 template <int MODE> struct ModeTraits;
-template <> struct ModeTraits<0> { using T = double; using V = v4d; static constexpr int BK = 16; static constexpr int PAD = 1; };
-template <> struct ModeTraits<1> { using T = float; using V = v4f; static constexpr int BK = 16; static constexpr int PAD = 1; };
+template <> struct ModeTraits<0> { using T = double; using V = v4d; static constexpr int BK = LRA_GEMM_BK64; static constexpr int PAD = 1; };
+template <> struct ModeTraits<1> { using T = float; using V = v4f; static constexpr int BK = LRA_GEMM_BK32; static constexpr int PAD = 1; };
 template <> struct ModeTraits<2> { using T = float; using V = v4f; static constexpr int BK = 32; static constexpr int PAD = 4; };
 
 __device__ __forceinline__ unsigned short f2bf_rne(float x) {
@@ -359,10 +365,11 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
         return;
     }
     if (b.atomic || !b.coltab.empty()) {
-        // column-table / atomic batches: plain A and B, 128x128 tiles
-        if (b.trans_a || b.trans_b || b.cfg != T128x128)
-            LRA_THROW("column-table GEMM: plain A and B, 128x128 tiles");
-        launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
+        // column-table / atomic batches: plain A and B, 64x64 or 128x128 tiles
+        if (b.trans_a || b.trans_b || (b.cfg != T128x128 && b.cfg != T64x64))
+            LRA_THROW("column-table GEMM: plain A and B, 64x64 or 128x128 tiles");
+        if (b.cfg == T64x64) launch_one<MODE, 0, false, false, 2>(b, A, B, C, alpha, beta, s);
+        else launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
         return;
     }
     if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);

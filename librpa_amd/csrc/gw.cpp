@@ -236,7 +236,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                                        (int)R0.lsum[J], R0.cwb_off[J], nb[J] * na[J], u_off[(size_t)J * nR + S],
                                        nb[J] * na[J]));
                 }
-            c.ub.finalize(T128x128, st);
+            c.ub.finalize(big_tile(prec), st);
             flops_sigma += c.ub.flops;
             uchunks.push_back(std::move(c));
             j = jj;
@@ -284,7 +284,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                 for (int l = 0; l < nb[L]; ++l) b2.coltab.push_back((long long)S * NAO + ao[L] + l);
             }
             b1.finalize(T32x128, st);
-            b2.finalize(T128x128, st);
+            b2.finalize(big_tile(prec), st);
             flops_sigma += b1.flops + b2.flops;
         }
     }
@@ -310,7 +310,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
     a3a.atomic = true;
     a3a.coltab = ext_cols;
     for (int I = 0; I < natom; ++I) add_a3(a3a, I, y_off[I]);
-    a3a.finalize(T128x128, st);
+    a3a.finalize(big_tile(prec), st);
     flops_sigma += a3a.flops;
 
     // ---- chunks of I for T^D ----
@@ -381,7 +381,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
             c.a3b.atomic = true;
             c.a3b.coltab = ext_cols;
             for (int I = c.i0; I < c.i1; ++I) add_a3(c.a3b, I, tdo(I) + (long long)na[I] * loff[I][onsite[I]] * LD);
-            c.a3b.finalize(T128x128, st);
+            c.a3b.finalize(big_tile(prec), st);
             flops_sigma += c.tdb.flops + c.dcon.flops + c.a3b.flops;
             ichunks.push_back(std::move(c));
             i = ii;
@@ -463,7 +463,7 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
             pm.kmode = 2;
             bm.add(pm);
         }
-        const int cfg = n2 >= 1024 ? T128x128 : T64x64;
+        const int cfg = T64x64;
         bx.finalize(cfg, st);
         bm.finalize(cfg, st);
         bx.flops = 2.0 * cnt * (double)n2 * n2 * n2 / 2.0;

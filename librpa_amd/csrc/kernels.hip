@@ -421,7 +421,7 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
                                      strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
                 }
                 tr.finalize(T64x64, s);
-                sy.finalize((long long)m * nbatch >= 4096 ? T128x128 : T64x64, s);
+                sy.finalize(T64x64, s);
             }
             cp.trsm.push_back(std::move(tr));
             cp.syrk.push_back(std::move(sy));

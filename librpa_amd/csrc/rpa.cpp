@@ -309,7 +309,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         }
         const bool bf = prec == Prec::BF16X3;
         (void)bf;
-        c.wb.finalize(T128x128, stream);
+        c.wb.finalize(big_tile(prec), stream);
         c.xb.trans_a = true;
         c.xb.finalize(T64x64, stream);
         c.ta.finalize(T64x128, stream);
@@ -427,7 +427,7 @@ void RpaEngine::prepare_gf(const MeanField &mf, int nbands_g) {
                 b.add(make_prob(2 * NAO, NAO, 2 * (hi - lo), (long long)k * 2 * nbands * 2 * NAO + 2LL * lo * 2 * NAO,
                                 2 * NAO, (long long)k * 2 * nbands * NAO + 2LL * lo * NAO, NAO,
                                 (long long)k * 2 * NAO * NAO, NAO));
-        b.finalize(T128x128, stream);
+        b.finalize(T64x64, stream);
     }
     gf_nbands = nbands_g;
 }
@@ -566,7 +566,7 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             pm.kmode = 2;  // A = L^T: row tile m0 needs k >= m0 only
             bm.add(pm);
         }
-        const int cfg = n2 >= 1024 ? T128x128 : T64x64;
+        const int cfg = n2 >= 1024 ? big_tile(ec_prec) : T64x64;
         bx.finalize(cfg, stream);
         bm.finalize(cfg, stream);
         // executed flops of the triangular products
