@@ -17,6 +17,7 @@
 //     offsets (a_k, b_k), for contractions whose summed index spans several strides.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -76,9 +77,18 @@ struct GemmBatch {
     double flops = 0;      // algorithmic flops of one launch (2 M N K summed)
     DevBuf d_probs, d_rowtab, d_coltab, d_ktab;
 
+    bool tuned = false;
+    float tuned_ms = 0;    // time of the chosen tile shape in the tuning run
+
     void add(const GemmProb &p) { probs.push_back(p); }
     void finalize(int cfg_, hipStream_t s = 0);
     bool empty() const { return probs.empty(); }
+    // Autotuner: time one launch of every tile shape this batch can use (after a warm-up
+    // launch) on the real operands and keep the fastest.  The launches must be harmless
+    // to repeat (overwriting outputs); `before` runs before every launch (e.g. to reset an
+    // accumulated output).  Returns the chosen tile configuration.
+    int tune(Prec prec, const void *A, const void *B, void *C, double alpha, double beta, hipStream_t s,
+             const std::function<void()> &before = nullptr);
 };
 
 GemmProb make_prob(int M, int N, int K, long long a_off, int lda, long long b_off, int ldb,

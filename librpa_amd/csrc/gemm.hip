@@ -318,9 +318,10 @@ void GemmBatch::finalize(int cfg_, hipStream_t s) {
     }
     total_tiles = t0;
     d_probs = upload(probs, s);
-    if (!rowtab.empty()) d_rowtab = upload(rowtab, s);
-    if (!coltab.empty()) d_coltab = upload(coltab, s);
-    if (!ktab.empty()) d_ktab = upload(ktab, s);
+    // the tables do not depend on the tile shape: upload once
+    if (!rowtab.empty() && d_rowtab.bytes() != rowtab.size() * sizeof(long long)) d_rowtab = upload(rowtab, s);
+    if (!coltab.empty() && d_coltab.bytes() != coltab.size() * sizeof(long long)) d_coltab = upload(coltab, s);
+    if (!ktab.empty() && d_ktab.bytes() != ktab.size() * sizeof(long long)) d_ktab = upload(ktab, s);
 }
 
 template <int MODE, int CFG, bool TA, bool TB, int VAR>
@@ -376,6 +377,39 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
     else if (b.trans_a) launch_cfg<MODE, true, false>(b, A, B, C, alpha, beta, s);
     else if (b.trans_b) launch_cfg<MODE, false, true>(b, A, B, C, alpha, beta, s);
     else launch_cfg<MODE, false, false>(b, A, B, C, alpha, beta, s);
+}
+
+int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alpha, double beta, hipStream_t s,
+                    const std::function<void()> &before) {
+    if (probs.empty() || atomic) return cfg;
+    std::vector<int> cands;
+    if (!ktab.empty()) cands = {T64x64};
+    else if (!coltab.empty()) cands = {T64x64, T128x128};
+    else for (int c = 0; c < NUM_TILE_CFG; ++c) cands.push_back(c);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bc = cfg;
+    for (int c : cands) {
+        finalize(c, s);
+        if (before) before();
+        gemm_run(prec, *this, A, B, C, alpha, beta, s);
+        if (before) before();
+        HIP_CHECK(hipEventRecord(e0, s));
+        gemm_run(prec, *this, A, B, C, alpha, beta, s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; bc = c; }
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    finalize(bc, s);
+    tuned = true;
+    tuned_ms = best;
+    return bc;
 }
 
 void gemm_run(Prec prec, const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,

@@ -472,6 +472,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         for (int s = 0; s < nspin; ++s) {
             build_gf(mf, tau, s);
             char *chi_base = d_chi0s.as<char>() + (size_t)it * E_tot * ts;
+            const bool tune_now = autotune && it == 0 && s == 0;
             for (auto &c : chunks) {
                 prof.begin("chi0_gather", stream);
                 if (prec == Prec::F64)
@@ -481,11 +482,19 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     launch_gather_cols<float>(d_Gpm.as<float>(), 2LL * NAO, d_Ghat.as<float>(), c.d_jobs.as<GatherJob>(),
                                               (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
                 prof.end("chi0_gather", stream, 0, 2.0 * c.ghat_elems * ts);
+                if (tune_now && !c.wb.tuned) c.wb.tune(prec, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
                 prof.begin("chi0_W", stream);
                 gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
                 prof.end("chi0_W", stream, c.wb.flops);
             }
             for (auto &c : chunks) {
+                if (tune_now && !c.ta.tuned) {
+                    // TA overwrites T; the contraction trials overwrite chi0s with beta = 0 before
+                    // the X term below resets it for this tau
+                    c.ta.tune(prec, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    c.fin.tune(prec, d_Cfin.get(), d_T.get(), chi_base, 1.0, 0.0, stream);
+                }
                 prof.begin("chi0_X", stream);
                 gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
                 prof.end("chi0_X", stream, c.xb.flops);
@@ -550,11 +559,12 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     std::vector<double> gp;
     DevBuf dgp;
     std::vector<double> Vh(nn);
-    std::map<int, std::pair<GemmBatch, GemmBatch>> batches;  // chunk size -> (X = D L, M = I - L^T X)
+    // (chunk size, order, precision) -> (X = D L, M = I - L^T X), kept with their tuned tiles
     auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
-        auto it = batches.find(cnt);
-        if (it != batches.end()) return it->second;
-        auto &pr = batches[cnt];
+        const auto key = std::make_tuple(cnt, n2, (int)ec_prec);
+        auto it = ec_batches.find(key);
+        if (it != ec_batches.end()) return it->second;
+        auto &pr = ec_batches[key];
         GemmBatch &bx = pr.first, &bm = pr.second;
         bm.trans_a = true;
         bm.lower_only = true;
@@ -656,6 +666,16 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             const double tbytes = (double)ts * E_tot * ntau_done * ceil_div(cnt, 8) + (double)es * cnt * nn;
             prof.end("chi0_transform", stream, 0, tbytes);
             auto &bb = get_batches(cnt);
+            if (autotune && !bb.first.tuned) {
+                bb.first.tune(ec_prec, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
+                gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
+                bb.second.tune(ec_prec, Lp, dX.get(), dM.get(), -1.0, 1.0, stream, [&]() {
+                    for (int w = 0; w < cnt; ++w) {
+                        if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream);
+                        else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream);
+                    }
+                });
+            }
             prof.begin("ec_gemm_x", stream);
             gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
             prof.end("ec_gemm_x", stream, bb.first.flops);

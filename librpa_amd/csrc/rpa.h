@@ -2,6 +2,8 @@
 #pragma once
 
 #include <array>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "dataset.h"
@@ -72,6 +74,8 @@ struct RpaEngine {
     double t_chi0 = 0, t_ec = 0;
     int ntau_done = 0;
     int planned_ntau = 0;
+    bool autotune = true;  // pick each batch's tile shape by timing on first use
+    std::map<std::tuple<int, int, int>, std::pair<GemmBatch, GemmBatch>> ec_batches;
 
     void plan(const Dataset &ds, Prec p, Prec pe, int ntau);
     // G(R, +-tau) of spin s into d_Gpm (prepare_gf once per band cut first)
