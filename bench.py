@@ -59,15 +59,39 @@ def run_steps(L, h, o, nk, prec, steps, warmup):
     return ec, times, L.profile(), h.get_stats()
 
 
-def dominant_kernel(prof, prec_name):
-    """Largest-time region among the single-kernel GEMM regions."""
+def _profiled_traffic(kernel, workload, prec_name):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
+    (profiles/<kernel>_<workload>_<prec>.md, FETCH_SIZE and WRITE_SIZE scaled by a copy
+    calibration), or None when no summary exists for this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{prec_name}.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def dominant_kernel(prof, prec_name, workload):
+    """Largest-time region among the single-kernel GEMM regions, placed on its roofline."""
     cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_X", "chi0_contract",
                                                             "ec_gemm"))}
-    name, (ms, calls, flops, _bytes) = max(cands.items(), key=lambda kv: kv[1][0])
+    name, (ms, calls, flops, nbytes) = max(cands.items(), key=lambda kv: kv[1][0])
+    peak_f = PEAK_TFLOPS[prec_name]
+    t_f = flops / (peak_f * 1e12)
+    t_b = nbytes / (HBM_GBS * 1e9)
+    traffic = _profiled_traffic(name, workload, prec_name)
+    common = dict(kernel=name, calls=int(calls), avg_ms=round(ms / calls, 4),
+                  flops_per_launch=flops / calls, alg_bytes_per_launch=nbytes / calls if nbytes else None,
+                  traffic=traffic)
+    if nbytes and t_b > t_f:
+        ach = nbytes / (ms * 1e-3) / 1e9
+        return name, dict(bound="hbm", achieved=round(ach, 1), peak=HBM_GBS, unit="GB/s", frac=round(ach / HBM_GBS, 4),
+                          **common)
     ach = flops / (ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[prec_name if name.startswith("chi0") else prec_name]
-    return name, dict(bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s", frac=round(ach / peak, 4),
-                      traffic=None, kernel=name, calls=int(calls), avg_ms=round(ms / calls, 4))
+    return name, dict(bound="mfma", achieved=round(ach, 3), peak=peak_f, unit="TFLOP/s", frac=round(ach / peak_f, 4),
+                      **common)
 
 
 def cpu_baseline(sys_, grid, nfreq):
@@ -113,6 +137,47 @@ def cpu_baseline(sys_, grid, nfreq):
                         f"by (n_abf/{m})^3; total / nfreq"))
 
 
+def copy_calib(nbytes):
+    """A plain copy of known size: reads and writes nbytes once each (for scaling HBM counters)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes": nbytes}), flush=True)
+
+
+def g0w0_bench(args):
+    """W(q, w) -> W(R, tau) and Sigma_c(R, i w) wall time on one GPU, per precision."""
+    import librpa_amd as L
+    from librpa_amd import synth
+    L.init_global()
+    wl = args.workload if args.workload != "cubic128" else "cubic64"
+    t0 = time.time()
+    s = synth.WORKLOADS[wl]()
+    _log(f"workload {wl}: {s.natom} atoms, n_ao {s.nbasis}, n_abf {s.nabf_tot}, nk {s.nk}, nfreq {s.nfreq} "
+         f"(generated in {time.time() - t0:.1f}s)")
+    h = L.Handler()
+    L.load_system(h, s, vq_cut=True)
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    precs = ["fp64"] if args.prec in ("auto", "fp64") else [args.prec]
+    res = {"metric": "G0W0 chi0 + W + Sigma_c wall s", "unit": "s", "higher_is_better": False, "n_gpus": 1,
+           "config": {"workload": wl, "atoms": s.natom, "n_ao": s.nbasis, "n_abf": s.nabf_tot, "nk": s.nk,
+                      "nfreq": s.nfreq}, "data": "synthetic (seeded generator, librpa_amd/synth.py)", "runs": {}}
+    for pn in precs:
+        h.set_precision(L.PREC[pn], L.PREC[pn])
+        L.profile_reset()
+        t0 = time.perf_counter()
+        h.build_g0w0_sigma(o)
+        t = time.perf_counter() - t0
+        prof = L.profile()
+        res["runs"][pn] = {"wall_s": round(t, 3), "regions_ms": {k: round(v[0], 2) for k, v in prof.items()}}
+        _log(f"{pn}: G0W0 build {t:.2f} s")
+    res["value"] = res["runs"][precs[0]]["wall_s"]
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,11 +188,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
                     help="where the synthetic Coulomb matrix product is formed (cpu under profilers)")
+    ap.add_argument("--task", default="rpa", choices=["rpa", "g0w0"],
+                    help="rpa: the chi0 + Ec metric (default); g0w0: W and Sigma_c timing on --workload")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
               "(no inter-GPU sharding in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib > 0:
+        copy_calib(args.copy_calib)
+        return
+    if args.task == "g0w0":
+        g0w0_bench(args)
+        return
 
     import librpa_amd as L
     from librpa_amd import synth
@@ -180,7 +255,7 @@ def main():
     pn, ec, times, prof, st, dec = chosen
     step = float(np.median(times))
     total_flops = st["flops_chi0_alg"]
-    kname, roof = dominant_kernel(prof, pn)
+    kname, roof = dominant_kernel(prof, pn, args.workload)
     res = {
         "metric": "chi0+RPA Ec wall s/freq",
         "value": round(step / s.nfreq, 5),

@@ -73,7 +73,8 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
 }
 
 // VAR selects the addressing extras compiled in: 0 plain, 1 K table, 2 column table and
-// atomic epilogue (kept out of the plain kernels, where they cost registers).
+// atomic epilogue (kept out of the plain kernels, where they cost registers), 3 plain code
+// under a separate symbol (the chi0 contraction, so that kernel traces can isolate it).
 template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB, int VAR>
 __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
                                                     const typename ModeTraits<MODE>::T *__restrict__ A,
@@ -340,17 +341,17 @@ static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C
     HIP_CHECK(hipGetLastError());
 }
 
-template <int MODE, bool TA, bool TB>
+template <int MODE, bool TA, bool TB, int VAR = 0>
 static void launch_cfg(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
                        hipStream_t s) {
     switch (b.cfg) {
-        case 0: launch_one<MODE, 0, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 1: launch_one<MODE, 1, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 2: launch_one<MODE, 2, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 3: launch_one<MODE, 3, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 4: launch_one<MODE, 4, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 5: launch_one<MODE, 5, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
-        case 6: launch_one<MODE, 6, TA, TB, 0>(b, A, B, C, alpha, beta, s); break;
+        case 0: launch_one<MODE, 0, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 1: launch_one<MODE, 1, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 2: launch_one<MODE, 2, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 3: launch_one<MODE, 3, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 4: launch_one<MODE, 4, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 5: launch_one<MODE, 5, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 6: launch_one<MODE, 6, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
         default: LRA_THROW("bad tile cfg");
     }
 }
@@ -371,6 +372,12 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
             LRA_THROW("column-table GEMM: plain A and B, 64x64 or 128x128 tiles");
         if (b.cfg == T64x64) launch_one<MODE, 0, false, false, 2>(b, A, B, C, alpha, beta, s);
         else launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
+        return;
+    }
+    if (b.tag == 3) {
+        // same code as the plain kernel, instantiated separately so that traces can tell it apart
+        if (b.trans_a || b.trans_b) LRA_THROW("tagged GEMM: plain A and B");
+        launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
         return;
     }
     if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);

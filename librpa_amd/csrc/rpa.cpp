@@ -313,10 +313,18 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.xb.trans_a = true;
         c.xb.finalize(T64x64, stream);
         c.ta.finalize(T64x128, stream);
+        c.fin.tag = 3;
         c.fin.finalize(T64x64, stream);
         c.d_jobs = upload(c.jobs, stream);
         c.d_colsrc = upload(c.colsrc, stream);
         flops_chi0_exec += c.wb.flops + c.xb.flops + c.ta.flops + c.fin.flops;
+        // algorithmic HBM bytes of one contraction launch: Cfin_I and T read once, chi0s read and written
+        c.fin_bytes = 0;
+        for (const auto &p : c.fin.probs)
+            c.fin_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + 2.0 * p.M * p.N);
+        c.ta_bytes = 0;
+        for (const auto &p : c.ta.probs)
+            c.ta_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + (double)p.M * p.N);
     }
     flops_chi0_exec *= ntau;
     size_t max_g = 0, max_t = 0;
@@ -500,10 +508,10 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 prof.end("chi0_X", stream, c.xb.flops);
                 prof.begin("chi0_TA", stream);
                 gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                prof.end("chi0_TA", stream, c.ta.flops);
+                prof.end("chi0_TA", stream, c.ta.flops, c.ta_bytes);
                 prof.begin("chi0_contract", stream);
                 gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
-                prof.end("chi0_contract", stream, c.fin.flops);
+                prof.end("chi0_contract", stream, c.fin.flops, c.fin_bytes);
             }
         }
     }
