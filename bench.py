@@ -160,7 +160,7 @@ def g0w0_bench(args):
          f"(generated in {time.time() - t0:.1f}s)")
     h = L.Handler()
     L.load_system(h, s, vq_cut=True)
-    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI, output_level=4)
     precs = ["fp64"] if args.prec in ("auto", "fp64") else [args.prec]
     res = {"metric": "G0W0 chi0 + W + Sigma_c wall s", "unit": "s", "higher_is_better": False, "n_gpus": 1,
            "config": {"workload": wl, "atoms": s.natom, "n_ao": s.nbasis, "n_abf": s.nabf_tot, "nk": s.nk,

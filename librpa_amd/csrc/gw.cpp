@@ -393,15 +393,9 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
     long long max_g = 0, max_td = 0;
     for (auto &c : uchunks) max_g = std::max(max_g, c.ghat_elems);
     for (auto &c : ichunks) max_td = std::max(max_td, c.td_elems);
-    d_U.alloc(u_elems * ts);
-    d_Ghat.alloc(max_g * ts);
-    d_TD.alloc(max_td * ts);
-    d_Y.alloc(y_elems * ts);
-    d_GC.alloc(gc_elems * ts);
-    for (int sg = 0; sg < 2; ++sg) d_sig[sg].alloc((size_t)nR * NAO * NAO * ts);
-    d_extA.alloc((size_t)NE * NAO * NAO * ts);
-    d_extB.alloc((size_t)NE * NAO * NAO * ts);
-    if (prec != Prec::F64) d_Wf.alloc((size_t)nR * NABF * NABF * ts);
+    // work buffers of the Sigma phase are allocated when it starts (after W's temporaries are freed)
+    max_ghat = max_g;
+    max_tdc = max_td;
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -519,6 +513,7 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
             snr[rr] = std::sin(ang);
         }
         DevBuf dcs = upload(csr, st), dsn = upload(snr, st);
+        if (verbose) { fprintf(stderr, "[librpa-mi355x] W: q %d / %d (%.1f s)\n", iq + 1, nk, now_s() - t0); fflush(stderr); }
         for (int w0 = 0; w0 < nw; w0 += WCH) {
             const int cnt = std::min(WCH, nw - w0);
             prof.begin("gw_w_chi0q", st);
@@ -603,6 +598,15 @@ void GwEngine::build_sigma(const Dataset &ds, const TFGrid &g) {
     HIP_CHECK(hipMemsetAsync(d_Sre.get(), 0, d_Sre.bytes(), st));
     HIP_CHECK(hipMemsetAsync(d_Sim.get(), 0, d_Sim.bytes(), st));
     if (w_zero) { HIP_CHECK(hipStreamSynchronize(st)); t_sigma = now_s() - t0; return; }
+    d_U.alloc(u_elems * ts);
+    d_Ghat.alloc(max_ghat * ts);
+    d_TD.alloc(max_tdc * ts);
+    d_Y.alloc(y_elems * ts);
+    d_GC.alloc(gc_elems * ts);
+    for (int sg = 0; sg < 2; ++sg) d_sig[sg].alloc((size_t)nR * NAO * NAO * ts);
+    d_extA.alloc((size_t)NE * NAO * NAO * ts);
+    d_extB.alloc((size_t)NE * NAO * NAO * ts);
+    if (prec != Prec::F64) d_Wf.alloc((size_t)nR * NABF * NABF * ts);
     R0.prepare_gf(ds.mf, -1);  // all bands (get_gf_cplx_imagtimes_Rs, meanfield.cpp:309-365)
     const long long gpm_r = (long long)NAO * 2 * NAO;
     (void)gpm_r;
@@ -623,6 +627,7 @@ void GwEngine::build_sigma(const Dataset &ds, const TFGrid &g) {
             sw[w] = g.sin_t2f[(size_t)w * g.n + it];
         }
         DevBuf dcw = upload(cw, st), dsw = upload(sw, st);
+        if (verbose) { fprintf(stderr, "[librpa-mi355x] Sigma_c: tau %d / %d (%.1f s)\n", it + 1, g.n, now_s() - t0); fflush(stderr); }
         for (int s = 0; s < nspin; ++s) {
             R0.build_gf(ds.mf, g.tau[it], s);
             for (int sg = 0; sg < 2; ++sg) {
@@ -698,6 +703,7 @@ void GwEngine::build_sigma(const Dataset &ds, const TFGrid &g) {
         }
         HIP_CHECK(hipStreamSynchronize(st));
     }
+    for (DevBuf *b : {&d_U, &d_Ghat, &d_TD, &d_Y, &d_GC, &d_sig[0], &d_sig[1], &d_extA, &d_extB, &d_Wf}) b->release();
     t_sigma = now_s() - t0;
 }
 
@@ -873,7 +879,9 @@ void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
         d.rpa = eng;
     }
     eng->build_chi0(d, d.grid, o.n_bands_chi0);
+    eng->release_work();  // chi0(R, tau) stays; the chi0 work buffers are rebuilt on demand
     auto gw = std::make_shared<GwEngine>();
+    gw->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
     gw->plan(d, eng);
     gw->build_w(d, d.grid);
     gw->build_sigma(d, d.grid);

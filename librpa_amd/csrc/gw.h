@@ -2,6 +2,7 @@
 // self-energy Sigma(R, i tau) = C G W C contraction on top of the chi0 engine.
 #pragma once
 
+#include <cstdio>
 #include <memory>
 #include <vector>
 
@@ -74,6 +75,8 @@ struct GwEngine {
     bool w_zero = false;  // no truncated Coulomb given: W = 0
 
     double flops_sigma = 0, flops_w = 0, t_w = 0, t_sigma = 0;
+    long long max_ghat = 0, max_tdc = 0;
+    bool verbose = false;  // progress lines on stderr
 
     void plan(const Dataset &ds, std::shared_ptr<RpaEngine> r);
     void build_w(const Dataset &ds, const TFGrid &g);

@@ -332,6 +332,8 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         max_g = std::max(max_g, (size_t)c.ghat_elems);
         max_t = std::max(max_t, (size_t)c.t_elems);
     }
+    ghat_max = max_g;
+    t_max = max_t;
     d_Ghat.alloc(max_g * ts);
     d_W.alloc((size_t)w_off.back() * ts);
     d_T.alloc(max_t * ts);
@@ -474,6 +476,9 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     const double t0 = now_s();
     const int ntau = g.n;
     prepare_gf(mf, nbands_g);
+    d_Ghat.reserve(ghat_max * ts);
+    d_W.reserve((size_t)w_off.back() * ts);
+    d_T.reserve(t_max * ts);
     HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
@@ -727,6 +732,12 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     for (auto &c : res.qcontrib) res.ec_re += c.real();
     t_ec = now_s() - t0;
     return res;
+}
+
+void RpaEngine::release_work() {
+    d_Ghat.release();
+    d_W.release();
+    d_T.release();
 }
 
 void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {

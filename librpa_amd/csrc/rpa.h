@@ -83,6 +83,8 @@ struct RpaEngine {
     void prepare_gf(const MeanField &mf, int nbands_g);
     void build_gf(const MeanField &mf, double tau, int s);
     void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);
+    void release_work();  // free the gathered-G, W and T work buffers (rebuilt by build_chi0)
+    long long ghat_max = 0, t_max = 0;
     RpaResult ec(const Dataset &ds, const TFGrid &g);
     // dense chi0(q, omega) block for verification
     void chi0q_block(const Dataset &ds, const TFGrid &g, int iw, int iq, int I, int J, double *re, double *im);
