@@ -1,0 +1,302 @@
+"""Driver for on-disk LibRPA inputs: ``python -m librpa_amd.driver [librpa.in]``.
+
+Mirrors LibRPA's driver for ``task = rpa`` (driver/main.cpp:104-212, driver/task_rpa.cpp):
+reads the SCF and RI data files of ``input_dir`` and pushes them through the C API, then prints
+``| Total EcRPA: %18.9f`` as the reference driver does.  File formats (docs/develop/
+dataset_format.md; readers in driver/read_data.cpp):
+
+* ``band_out`` (read_scf_occ_eigenvalues :32-140): n_k, n_spin, n_states, n_basis, E_F, then per
+  k and spin a "ik is" line and n_states lines "i occ e_Ha e_eV";
+* ``stru_out`` (read_stru :1814-1867): 3 lattice rows, 3 reciprocal rows, n_atoms, then
+  "x y z type" per atom (Cartesian bohr, 1-based type);
+* ``bz_sampling_out`` (read_bz_sampling :1869-1916): n1 n2 n3, n_full n_ibz, then per k two
+  indices, three fractional values, the Cartesian k (bohr^-1), and "weight ibz-index";
+* ``basis_out`` (read_basis :1976-2021): n_types, total NAO, total ABF, code name, then
+  "type n_ao n_abf" per type;
+* ``KS_eigenvector_*`` (handle_KS_file :184-284): blocks "ik" followed by re/im pairs ordered
+  basis function (outer), state, spin (inner);
+* ``Cs_data_*`` (handle_Cs_file{,_binary} :618-748): ASCII or unformatted; per block I, J (1-based),
+  R, n_i, n_j, n_mu and n_i*n_j*n_mu values in (i, j, mu) order; blocks whose largest |value| is
+  below ``cs_threshold`` (default 1e-6, driver/driver.cpp:16) are dropped;
+* ``coulomb_mat_*`` (handle_Vq_row_file :1472-1668): ASCII or unformatted row blocks
+  (n_abf, row range, column range, k index, k weight) of V(q); atom-pair blocks I <= J are passed on.
+The binary/ASCII choice follows check_Cs_file_binary / check_coulomb_file_binary (:415-454): a file
+whose first token does not parse as an integer is binary.
+"""
+from __future__ import annotations
+
+import os
+import re
+import struct
+import sys
+from typing import Dict, List
+
+import numpy as np
+
+
+def parse_input(path: str) -> Dict[str, str]:
+    """``key = value`` lines, ``#`` comments (driver/inputfile.cpp)."""
+    out = {}
+    for line in open(path):
+        line = line.split('#')[0].strip()
+        if '=' in line:
+            k, v = line.split('=', 1)
+            out[k.strip().lower()] = v.strip()
+    return out
+
+
+def _tokens(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def _is_binary(path):
+    with open(path, 'rb') as f:
+        head = f.read(64)
+    try:
+        tok = head.decode('ascii').split()
+        int(tok[0])
+        return False
+    except (UnicodeDecodeError, ValueError, IndexError):
+        return True
+
+
+def read_band(path):
+    t = _tokens(path)
+    nk, ns, nst, nb = int(t[0]), int(t[1]), int(t[2]), int(t[3])
+    ef = float(t[4])
+    occ = np.zeros((ns, nk, nst))
+    eps = np.zeros((ns, nk, nst))
+    p = 5
+    for _ in range(nk):
+        for s in range(ns):
+            k = int(t[p]) - 1
+            p += 2
+            for i in range(nst):
+                occ[s, k, i] = float(t[p + 1])
+                eps[s, k, i] = float(t[p + 2])
+                p += 4
+    return dict(nk=nk, nspin=ns, nstates=nst, nbasis=nb, efermi=ef, occ=occ, eps=eps)
+
+
+def read_stru(path):
+    t = _tokens(path)
+    lat = np.array([float(x) for x in t[0:9]]).reshape(3, 3)
+    G = np.array([float(x) for x in t[9:18]]).reshape(3, 3)
+    na = int(t[18])
+    pos = np.zeros((na, 3))
+    types = np.zeros(na, dtype=np.int64)
+    p = 19
+    for a in range(na):
+        pos[a] = [float(x) for x in t[p:p + 3]]
+        types[a] = int(t[p + 3]) - 1
+        p += 4
+    return dict(latvec=lat, G=G, positions=pos, types=types)
+
+
+def read_bz(path):
+    t = _tokens(path)
+    n = [int(x) for x in t[0:3]]
+    nfull = int(t[3])
+    p = 5
+    kv = np.zeros((nfull, 3))
+    ibz = np.zeros(nfull, dtype=np.int64)
+    for i in range(nfull):
+        p += 5
+        kv[i] = [float(x) for x in t[p:p + 3]]
+        p += 3
+        ibz[i] = int(t[p + 1]) - 1
+        p += 2
+    return dict(period=tuple(n), kcart=kv, ibz=ibz)
+
+
+def read_basis(path, types):
+    t = _tokens(path)
+    nt = int(t[0])
+    m_ao, m_abf = {}, {}
+    p = 4
+    for _ in range(nt):
+        ty, a, b = int(t[p]) - 1, int(t[p + 1]), int(t[p + 2])
+        m_ao[ty], m_abf[ty] = a, b
+        p += 3
+    return (np.array([m_ao[int(x)] for x in types], dtype=np.int64),
+            np.array([m_abf[int(x)] for x in types], dtype=np.int64))
+
+
+def read_eigenvectors(dir_path, nspin, nk, nst, nb):
+    wfc = np.zeros((nspin, nk, nst, nb), dtype=np.complex128)
+    files = sorted(f for f in os.listdir(dir_path) if f.startswith("KS_eigenvector"))
+    if not files:
+        raise RuntimeError("no KS_eigenvector files in " + dir_path)
+    for fn in files:
+        t = _tokens(os.path.join(dir_path, fn))
+        p = 0
+        while p < len(t):
+            k = int(t[p]) - 1
+            p += 1
+            vals = np.array([float(x) for x in t[p:p + 2 * nb * nst * nspin]])
+            p += 2 * nb * nst * nspin
+            v = (vals[0::2] + 1j * vals[1::2]).reshape(nb, nst, nspin)   # (basis, state, spin)
+            wfc[:, k] = v.transpose(2, 1, 0)
+    return wfc
+
+
+def read_cs(dir_path, threshold=1e-6, prefix="Cs_data"):
+    out = []
+    files = sorted(f for f in os.listdir(dir_path) if f.startswith(prefix))
+    for fn in files:
+        path = os.path.join(dir_path, fn)
+        if _is_binary(path):
+            data = open(path, 'rb').read()
+            natom, ncell, nblk = struct.unpack_from('3i', data, 0)
+            p = 12
+            for _ in range(nblk):
+                d = struct.unpack_from('8i', data, p)
+                p += 32
+                ni, nj, nmu = d[5], d[6], d[7]
+                blk = np.frombuffer(data, dtype=np.float64, count=ni * nj * nmu, offset=p).copy()
+                p += 8 * ni * nj * nmu
+                if np.abs(blk).max() >= threshold:
+                    out.append((d[0] - 1, d[1] - 1, (d[2], d[3], d[4]), blk.reshape(ni * nj, nmu)))
+        else:
+            t = _tokens(path)
+            p = 2
+            while p + 8 <= len(t):
+                ia, ja = int(t[p]) - 1, int(t[p + 1]) - 1
+                R = (int(t[p + 2]), int(t[p + 3]), int(t[p + 4]))
+                ni, nj, nmu = int(t[p + 5]), int(t[p + 6]), int(t[p + 7])
+                p += 8
+                blk = np.array([float(x) for x in t[p:p + ni * nj * nmu]])
+                p += ni * nj * nmu
+                if np.abs(blk).max() >= threshold:
+                    out.append((ia, ja, R, blk.reshape(ni * nj, nmu)))
+    return out
+
+
+def read_coulomb(dir_path, nabf_tot, nk, prefix="coulomb_mat"):
+    """Dense V(q) per k index from the row-block files (all blocks present in the files)."""
+    V = np.zeros((nk, nabf_tot, nabf_tot), dtype=np.complex128)
+    files = sorted(f for f in os.listdir(dir_path) if f.startswith(prefix))
+    for fn in files:
+        path = os.path.join(dir_path, fn)
+        if _is_binary(path):
+            data = open(path, 'rb').read()
+            nirk, nloc = struct.unpack_from('2i', data, 0)
+            p = 8
+            for _ in range(nloc):
+                nbb, br, er, bc, ec, iq = struct.unpack_from('6i', data, p)
+                p += 24 + 8
+                nr, nc = er - br + 1, ec - bc + 1
+                vals = np.frombuffer(data, dtype=np.float64, count=2 * nr * nc, offset=p)
+                p += 16 * nr * nc
+                V[iq - 1, br - 1:er, bc - 1:ec] = (vals[0::2] + 1j * vals[1::2]).reshape(nr, nc)
+        else:
+            t = _tokens(path)
+            p = 1
+            while p + 7 <= len(t):
+                br, er, bc, ec = int(t[p + 1]), int(t[p + 2]), int(t[p + 3]), int(t[p + 4])
+                iq = int(t[p + 5])
+                p += 7
+                nr, nc = er - br + 1, ec - bc + 1
+                vals = np.array([float(x) for x in t[p:p + 2 * nr * nc]])
+                p += 2 * nr * nc
+                V[iq - 1, br - 1:er, bc - 1:ec] = (vals[0::2] + 1j * vals[1::2]).reshape(nr, nc)
+    return V
+
+
+def read_dataset(dir_path, cs_threshold=1e-6) -> dict:
+    """All inputs of a LibRPA dataset directory as arrays."""
+    j = lambda f: os.path.join(dir_path, f)
+    band = read_band(j("band_out"))
+    stru = read_stru(j("stru_out"))
+    bz = read_bz(j("bz_sampling_out"))
+    nao, nabf = read_basis(j("basis_out"), stru["types"])
+    wfc = read_eigenvectors(dir_path, band["nspin"], band["nk"], band["nstates"], band["nbasis"])
+    cs = read_cs(dir_path, cs_threshold)
+    V = read_coulomb(dir_path, int(nabf.sum()), band["nk"])
+    return dict(band=band, stru=stru, bz=bz, nao=nao, nabf=nabf, wfc=wfc, cs=cs, vq=V)
+
+
+def push(h, ds: dict, routing: int):
+    """The driver's setter sequence (driver/main.cpp:124-181)."""
+    b, st, bz = ds["band"], ds["stru"], ds["bz"]
+    h.set_scf_dimension(b["nspin"], b["nk"], b["nstates"], b["nbasis"])
+    h.set_wg_ekb_efermi(b["nspin"], b["nk"], b["nstates"], b["occ"].reshape(-1), b["eps"].reshape(-1),
+                        b["efermi"])
+    h.set_latvec_and_G(st["latvec"].reshape(9), st["G"].reshape(9))
+    h.set_atoms(st["types"], st["positions"])
+    h.set_kgrids_kvec(*bz["period"], bz["kcart"])
+    h.set_ibz_mapping(bz["ibz"])
+    h.set_ao_basis_wfc(ds["nao"])
+    h.set_ao_basis_aux(ds["nabf"])
+    for s in range(b["nspin"]):
+        for k in range(b["nk"]):
+            w = ds["wfc"][s, k]
+            h.set_wfc(s, k, b["nstates"], b["nbasis"], w.real, w.imag)
+    for (I, J, R, blk) in ds["cs"]:
+        h.set_lri_coeff(routing, I, J, int(ds["nao"][I]), int(ds["nao"][J]), int(ds["nabf"][I]), R, blk)
+    off = np.concatenate([[0], np.cumsum(ds["nabf"])])
+    nat = len(ds["nao"])
+    for ik in range(b["nk"]):
+        for I in range(nat):
+            for J in range(I, nat):
+                blk = ds["vq"][ik, off[I]:off[I + 1], off[J]:off[J + 1]]
+                h.set_aux_bare_coulomb_k_atom_pair(ik, I, J, blk.shape[0], blk.shape[1], blk.real, blk.imag, 0.0)
+
+
+def to_system(ds: dict, nfreq: int):
+    """The dataset as a librpa_amd.synth.System (for checking against the oracle)."""
+    from . import synth
+    b, st, bz = ds["band"], ds["stru"], ds["bz"]
+    lat = st["latvec"]
+    kfrac = bz["kcart"] @ lat.T / (2 * np.pi)
+    kfrac[np.abs(kfrac) < 1e-12] = 0.0
+    # Hermitian V(q) from the I <= J atom-pair blocks, as the library receives it
+    off = np.concatenate([[0], np.cumsum(ds["nabf"])])
+    nat = len(ds["nao"])
+    V = np.zeros_like(ds["vq"])
+    for I in range(nat):
+        for J in range(I, nat):
+            blk = ds["vq"][:, off[I]:off[I + 1], off[J]:off[J + 1]]
+            V[:, off[I]:off[I + 1], off[J]:off[J + 1]] = blk
+            if J != I:
+                V[:, off[J]:off[J + 1], off[I]:off[I + 1]] = blk.conj().transpose(0, 2, 1)
+    return synth.System(name="dataset", latvec=lat, positions=st["positions"], types=st["types"],
+                        nao=ds["nao"], nabf=ds["nabf"], period=bz["period"], kfrac=kfrac, eps=b["eps"],
+                        occ=b["occ"], efermi=b["efermi"], wfc=ds["wfc"], cs=ds["cs"], vq=V, nfreq=nfreq,
+                        nspin=b["nspin"])
+
+
+ROUTING = {"auto": 0, "rtau": 1, "atompair": 3, "libri": 2}
+
+
+def main(argv: List[str]) -> int:
+    import librpa_amd as L
+    path = argv[1] if len(argv) > 1 else "librpa.in"
+    p = parse_input(path)
+    task = p.get("task", "rpa").lower()
+    if task != "rpa":
+        print(f"task = {task} is outside this driver (task = rpa only)", file=sys.stderr)
+        return 2
+    d = p.get("input_dir", ".")
+    if not os.path.isabs(d):
+        d = os.path.join(os.path.dirname(os.path.abspath(path)), d)
+    d = d.rstrip('/') + '/'
+    ds = read_dataset(d, float(p.get("cs_threshold", "1e-6")))
+    L.init_global()
+    h = L.Handler()
+    routing = ROUTING.get(p.get("parallel_routing", "libri").lower(), 2)
+    push(h, ds, routing)
+    o = L.Options(nfreq=int(p.get("nfreq", "16")), tfgrids_type=L.MINIMAX, parallel_routing=routing)
+    nibz = len(set(int(x) for x in ds["bz"]["ibz"]))
+    ec, contrib = h.get_rpa_correlation_energy(o, nibz)
+    print("RPA correlation energy (Hartree)")
+    print("| Weighted contribution from each k:")
+    for i, c in enumerate(contrib):
+        print(f"| {i}: ({c.real},{c.imag})")
+    print(f"| Total EcRPA: {ec:18.9f}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))
