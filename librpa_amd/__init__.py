@@ -390,10 +390,11 @@ class Handler:
         return dict(zip(keys, out.tolist()))
 
 
-def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut: bool = False):
+def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut=False):
     """Push a librpa_amd.synth.System through the C API (mirrors the driver's read_* calls).
 
-    ``vq_cut`` also passes the Coulomb matrix as the truncated one used for W (G0W0)."""
+    ``vq_cut``: True passes the Coulomb matrix again as the truncated one used for W (G0W0);
+    an array (nk, n_abf, n_abf) passes that matrix instead."""
     ns, nk, nb, nbas = sys.nspin, sys.nk, sys.nbands, sys.nbasis
     h.set_scf_dimension(ns, nk, nb, nbas)
     h.set_wg_ekb_efermi(ns, nk, nb, sys.occ.reshape(-1), sys.eps.reshape(-1), sys.efermi)
@@ -416,5 +417,8 @@ def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut: 
                 for J in range(I, sys.natom):
                     b = V[off[I]:off[I + 1], off[J]:off[J + 1]]
                     h.set_aux_bare_coulomb_k_atom_pair(ik, I, J, b.shape[0], b.shape[1], b.real, b.imag, 0.0)
-                    if vq_cut:
+                    if isinstance(vq_cut, np.ndarray):
+                        c = vq_cut[ik][off[I]:off[I + 1], off[J]:off[J + 1]]
+                        h.set_aux_cut_coulomb_k_atom_pair(ik, I, J, c.shape[0], c.shape[1], c.real, c.imag, 0.0)
+                    elif vq_cut:
                         h.set_aux_cut_coulomb_k_atom_pair(ik, I, J, b.shape[0], b.shape[1], b.real, b.imag, 0.0)

@@ -402,6 +402,85 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
 // ------------------------------------------------------------------------------------------
 // W(q, omega) -> W(R, tau)
 // ------------------------------------------------------------------------------------------
+namespace {
+// Principal square root of a symmetric positive-definite n x n matrix A (in place) by the coupled
+// Newton-Schulz iteration Y <- Y T, Z <- T Z, T = (3 I - Z Y) / 2 from Y = A / c, Z = I with
+// c = ||A||_F >= lambda_max; Y -> (A / c)^{1/2}.  Equals the eigen-decomposition root of
+// power_hemat (src/math/matrix_m.h:916-970) for positive-definite input.  Runs to convergence.
+void ns_sqrt(double *A, int n, hipStream_t st) {
+    const size_t nn = (size_t)n * n;
+    DevBuf dY(nn * 8), dZ(nn * 8), dT(nn * 8), dP(nn * 8), dYn(nn * 8), dZn(nn * 8), dr(64);
+    GemmBatch b;
+    b.add(make_prob(n, n, n, 0, n, 0, n, 0, n));
+    b.finalize(T64x64, st);
+    launch_sumsq(A, (long long)nn, dr.as<double>(), st);
+    double h[2];
+    HIP_CHECK(hipMemcpyAsync(h, dr.get(), 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const double c = std::sqrt(h[0]);
+    if (!(c > 0) || !std::isfinite(c)) LRA_THROW("Coulomb matrix is zero or not finite");
+    launch_scale_add_identity(A, dY.as<double>(), n, 1.0 / c, 0.0, st);
+    launch_scale_add_identity(A, dZ.as<double>(), n, 0.0, 1.0, st);
+    double prev = 1e300;
+    int stall = 0;
+    for (int k = 0; k < 200; ++k) {
+        gemm_run(Prec::F64, b, dZ.get(), dY.get(), dP.get(), 1.0, 0.0, st);
+        launch_scale_add_identity(dP.as<double>(), dT.as<double>(), n, -0.5, 1.5, st);
+        gemm_run(Prec::F64, b, dY.get(), dT.get(), dYn.get(), 1.0, 0.0, st);
+        gemm_run(Prec::F64, b, dT.get(), dZ.get(), dZn.get(), 1.0, 0.0, st);
+        launch_absmax_diff(dYn.as<double>(), dY.as<double>(), (long long)nn, dr.as<double>(), st);
+        HIP_CHECK(hipMemcpyAsync(h, dr.get(), 16, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        std::swap(dY, dYn);
+        std::swap(dZ, dZn);
+        const double rel = h[0] / std::max(h[1], 1e-300);
+        if (!std::isfinite(rel)) LRA_THROW("square root of the Coulomb matrix did not converge (not positive definite?)");
+        // quadratic phase ends at rounding level: stop once the update no longer shrinks
+        if (rel < 1e-13) { if (rel >= prev || ++stall >= 3) break; }
+        prev = rel;
+        if (k == 199) LRA_THROW("square root of the Coulomb matrix did not converge");
+    }
+    launch_scale_add_identity(dY.as<double>(), A, n, std::sqrt(c), 0.0, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// dense (embedded when complex) Hermitian matrix of atom-pair blocks I <= J for k point kq
+void dense_coulomb(const RpaEngine &R0, const std::map<std::tuple<int, int, int>, std::vector<cplx>> &store, int kq,
+                   bool embed, bool real_part, std::vector<double> &Vh, double &imax, double &vmax) {
+    const int n = R0.NABF, n2 = embed ? 2 * n : n;
+    Vh.assign((size_t)n2 * n2, 0.0);
+    imax = vmax = 0;
+    for (const auto &kv : store) {
+        const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
+        if (ik != kq || I > J) continue;
+        const auto &blk = kv.second;
+        for (int mu = 0; mu < R0.na[I]; ++mu)
+            for (int nu = 0; nu < R0.na[J]; ++nu) {
+                const cplx v = blk[(size_t)mu * R0.na[J] + nu];
+                vmax = std::max(vmax, std::abs(v));
+                imax = std::max(imax, std::fabs(v.imag()));
+                const size_t r = R0.abf[I] + mu, c = R0.abf[J] + nu;
+                const double vi = real_part ? 0.0 : v.imag();
+                if (!embed) {
+                    Vh[r * n2 + c] = v.real();
+                    if (I != J) Vh[c * n2 + r] = v.real();
+                } else {
+                    Vh[r * n2 + c] = v.real();
+                    Vh[(n + r) * n2 + n + c] = v.real();
+                    Vh[(n + r) * n2 + c] = vi;
+                    Vh[r * n2 + n + c] = -vi;
+                    if (I != J) {
+                        Vh[c * n2 + r] = v.real();
+                        Vh[(n + c) * n2 + n + r] = v.real();
+                        Vh[(n + c) * n2 + r] = -vi;
+                        Vh[c * n2 + n + r] = vi;
+                    }
+                }
+            }
+    }
+}
+}  // namespace
+
 void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     auto &prof = profiler();
     const double t0 = now_s();
@@ -415,17 +494,18 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     // no truncated Coulomb: v_c^{1/2} = 0 and W = 0 (epsilon.cpp:1862-1879)
     w_zero = ds.vq_cut.empty();
     if (w_zero) { HIP_CHECK(hipStreamSynchronize(st)); t_w = now_s() - t0; return; }
-    for (const auto &kv : ds.vq) {
-        auto it = ds.vq_cut.find(kv.first);
-        bool same = it != ds.vq_cut.end() && it->second.size() == kv.second.size();
-        for (size_t i = 0; same && i < kv.second.size(); ++i)
-            same = std::abs(it->second[i] - kv.second[i]) <= 1e-12 * (1.0 + std::abs(kv.second[i]));
-        if (!same) LRA_THROW("W needs the truncated Coulomb matrix equal to the bare one (v_c = v) in this build");
-    }
-    if (ds.vq_cut.size() != ds.vq.size()) LRA_THROW("W needs the truncated Coulomb matrix equal to the bare one");
     if (!ds.ibz_map.empty())
         for (int k = 0; k < nk; ++k)
             if (ds.ibz_map[k] != k) LRA_THROW("G0W0 needs the full k grid (no IBZ reduction) in this build");
+    // v_c = v lets W = L M^{-1} L^T - v use the Cholesky factor of v; otherwise the square roots
+    // of v and v_c are formed (Newton-Schulz) and W = v_c^{1/2} eps^{-1} v_c^{1/2} - v_c
+    bool same_cut = ds.vq_cut.size() == ds.vq.size();
+    for (const auto &kv : ds.vq) {
+        auto it = ds.vq_cut.find(kv.first);
+        same_cut = same_cut && it != ds.vq_cut.end() && it->second.size() == kv.second.size();
+        for (size_t i = 0; same_cut && i < kv.second.size(); ++i)
+            same_cut = std::abs(it->second[i] - kv.second[i]) <= 1e-12 * (1.0 + std::abs(kv.second[i]));
+    }
 
     const bool embed = nk > 1;
     const int n2 = embed ? 2 * n : n;
@@ -434,13 +514,14 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const double per_w = 8.0 * (NN + 2.0 * nn);
-    const double budget = std::min(160e9, 0.7 * (double)free_b) - 8.0 * 3 * nn;
+    const double budget = std::min(160e9, 0.7 * (double)free_b) - 8.0 * 10 * nn;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
     DevBuf dV0(nn * 8), dL(nn * 8), dD(nn * 8 * WCH), dX(nn * 8 * WCH), dA(NN * 8 * WCH), dacc(sizeof(double) * (WCH + 8));
     int *dflag = reinterpret_cast<int *>(dacc.as<double>() + WCH + 4);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), st));
     CholWork cw;
-    std::vector<double> Vh(nn);
+    DevBuf dSc;
+    std::vector<double> Vh, Vch;
     std::map<int, std::pair<GemmBatch, GemmBatch>> batches;
     auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
         auto it = batches.find(cnt);
@@ -451,10 +532,10 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
         bm.lower_only = true;
         for (int w = 0; w < cnt; ++w) {
             GemmProb px = make_prob(n2, n2, n2, (long long)w * nn, n2, 0, n2, (long long)w * nn, n2);
-            px.kmode = 1;
+            px.kmode = same_cut ? 1 : 0;  // triangular Cholesky factor, or the full square root
             bx.add(px);
             GemmProb pm = make_prob(n2, n2, n2, 0, n2, (long long)w * nn, n2, (long long)w * NN, N);
-            pm.kmode = 2;
+            pm.kmode = same_cut ? 2 : 0;
             bm.add(pm);
         }
         const int cfg = T64x64;
@@ -468,44 +549,33 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     for (int iq = 0; iq < nk; ++iq) {
         const double *kf = &ds.kfrac[(size_t)iq * 3];
         const bool gamma = std::fabs(kf[0]) < 1e-12 && std::fabs(kf[1]) < 1e-12 && std::fabs(kf[2]) < 1e-12;
-        std::fill(Vh.begin(), Vh.end(), 0.0);
         double vmax = 0, imax = 0;
-        for (const auto &kv : ds.vq) {
-            const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
-            if (ik != iq || I > J) continue;
-            const auto &blk = kv.second;
-            for (int mu = 0; mu < R0.na[I]; ++mu)
-                for (int nu = 0; nu < R0.na[J]; ++nu) {
-                    const cplx v = blk[(size_t)mu * R0.na[J] + nu];
-                    vmax = std::max(vmax, std::abs(v));
-                    imax = std::max(imax, std::fabs(v.imag()));
-                    const size_t r = R0.abf[I] + mu, c = R0.abf[J] + nu;
-                    const double vi = gamma ? 0.0 : v.imag();
-                    if (!embed) {
-                        Vh[r * n2 + c] = v.real();
-                        if (I != J) Vh[c * n2 + r] = v.real();
-                    } else {
-                        Vh[r * n2 + c] = v.real();
-                        Vh[(n + r) * n2 + n + c] = v.real();
-                        Vh[(n + r) * n2 + c] = vi;
-                        Vh[r * n2 + n + c] = -vi;
-                        if (I != J) {
-                            Vh[c * n2 + r] = v.real();
-                            Vh[(n + c) * n2 + n + r] = v.real();
-                            Vh[(n + c) * n2 + r] = -vi;
-                            Vh[c * n2 + n + r] = vi;
-                        }
-                    }
-                }
+        // v^{1/2} from Re v at q = 0 (power_hemat_blacs_real, epsilon.cpp:2019-2024)
+        dense_coulomb(R0, ds.vq, iq, embed, gamma, Vh, imax, vmax);
+        if (same_cut) {
+            // with v_c = v the Cholesky identity needs one matrix for both roots: real v(0)
+            if (gamma && imax > 1e-12 * std::max(vmax, 1e-300))
+                LRA_THROW("W at q = 0 with v_c = v needs a real Coulomb matrix (Im v(0) != 0)");
+            HIP_CHECK(hipMemcpyAsync(dV0.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(dL.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
+            prof.begin("gw_w_chol_v", st);
+            cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st);
+            prof.end("gw_w_chol_v", st, (double)n2 * n2 * n2 / 3.0);
+        } else {
+            // dL <- v^{1/2}; dV0 <- v_c, dSc <- v_c^{1/2} (complex v_c kept, as the reference does)
+            double ic = 0, vc = 0;
+            dense_coulomb(R0, ds.vq_cut, iq, embed, false, Vch, ic, vc);
+            if (!embed && ic > 1e-12 * std::max(vc, 1e-300))
+                LRA_THROW("a complex truncated Coulomb matrix at Gamma-only sampling is outside this build");
+            prof.begin("gw_w_sqrt_v", st);
+            HIP_CHECK(hipMemcpyAsync(dL.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, st));
+            ns_sqrt(dL.as<double>(), n2, st);
+            HIP_CHECK(hipMemcpyAsync(dV0.get(), Vch.data(), nn * 8, hipMemcpyHostToDevice, st));
+            dSc.reserve(nn * 8);
+            HIP_CHECK(hipMemcpyAsync(dSc.get(), Vch.data(), nn * 8, hipMemcpyHostToDevice, st));
+            ns_sqrt(dSc.as<double>(), n2, st);
+            prof.end("gw_w_sqrt_v", st);
         }
-        // at q = 0 LibRPA takes v^{1/2} from Re v but keeps v_c complex; both agree only for real v(0)
-        if (gamma && imax > 1e-12 * std::max(vmax, 1e-300))
-            LRA_THROW("W at q = 0 needs a real Coulomb matrix in this build (Im v(0) != 0)");
-        HIP_CHECK(hipMemcpyAsync(dV0.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(dL.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
-        prof.begin("gw_w_chol_v", st);
-        cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st);
-        prof.end("gw_w_chol_v", st, (double)n2 * n2 * n2 / 3.0);
         std::vector<double> csr(nR), snr(nR);
         for (int rr = 0; rr < nR; ++rr) {
             const double ang = 2 * M_PI * (kf[0] * cells[rr][0] + kf[1] * cells[rr][1] + kf[2] * cells[rr][2]);
@@ -549,7 +619,7 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
             for (int w = 0; w < cnt; ++w) {
                 double *A = dA.as<double>() + (size_t)w * NN;
                 launch_set_identity<double>(A, n2, N, st);
-                launch_copy_block(dL.as<double>(), n2, A + (size_t)n2 * N, N, n2, n2, st);
+                launch_copy_block(same_cut ? dL.as<double>() : dSc.as<double>(), n2, A + (size_t)n2 * N, N, n2, n2, st);
                 launch_copy_block(dV0.as<double>(), n2, A + (size_t)n2 * N + n2, N, n2, n2, st);
             }
             gemm_run(Prec::F64, bb.second, dL.get(), dX.get(), dA.get(), -1.0, 1.0, st);

@@ -33,4 +33,12 @@ void launch_sigma_freq(const T *sp, const T *sn, long long nelem, const double *
 void launch_sigma_ks_rows(const double *sre, const double *sim, int nw, int nR, int n, const int *atom_of,
                           int natom, const double *ph, const double *U, int nst, double *Y, hipStream_t s);
 
+// Dense helpers of the Newton-Schulz square root (FP64, row-major n x n, lda = n):
+// B = alpha A + beta I
+void launch_scale_add_identity(const double *A, double *B, int n, double alpha, double beta, hipStream_t s);
+// out[0] = max |A - B|, out[1] = max |B| (single workgroup, overwritten)
+void launch_absmax_diff(const double *A, const double *B, long long n, double *out, hipStream_t s);
+// out[0] = sum A^2 (overwritten)
+void launch_sumsq(const double *A, long long n, double *out, hipStream_t s);
+
 }  // namespace lra

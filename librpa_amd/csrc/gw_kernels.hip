@@ -168,4 +168,60 @@ void launch_sigma_ks_rows(const double *sre, const double *sim, int nw, int nR, 
     HIP_CHECK(hipGetLastError());
 }
 
+__global__ void scale_add_identity_kernel(const double *__restrict__ A, double *__restrict__ B, int n, double alpha,
+                                          double beta) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    B[idx] = alpha * A[idx] + (i == j ? beta : 0.0);
+}
+void launch_scale_add_identity(const double *A, double *B, int n, double alpha, double beta, hipStream_t s) {
+    hipLaunchKernelGGL(scale_add_identity_kernel, dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, B, n, alpha,
+                       beta);
+    HIP_CHECK(hipGetLastError());
+}
+
+// single workgroup of 1024 threads: deterministic reductions without atomics
+__global__ __launch_bounds__(1024) void absmax_diff_kernel(const double *__restrict__ A, const double *__restrict__ B,
+                                                           long long n, double *out) {
+    __shared__ double r0[1024], r1[1024];
+    double d = 0, m = 0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        d = fmax(d, fabs(A[i] - B[i]));
+        m = fmax(m, fabs(B[i]));
+    }
+    r0[threadIdx.x] = d;
+    r1[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + o]);
+            r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = r0[0]; out[1] = r1[0]; }
+}
+void launch_absmax_diff(const double *A, const double *B, long long n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(absmax_diff_kernel, dim3(1), dim3(1024), 0, s, A, B, n, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(1024) void sumsq_kernel(const double *__restrict__ A, long long n, double *out) {
+    __shared__ double r[1024];
+    double v = 0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) v += A[i] * A[i];
+    r[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) r[threadIdx.x] += r[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = r[0];
+}
+void launch_sumsq(const double *A, long long n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(1), dim3(1024), 0, s, A, n, out);
+    HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace lra

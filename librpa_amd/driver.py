@@ -217,8 +217,24 @@ def read_dataset(dir_path, cs_threshold=1e-6) -> dict:
     return dict(band=band, stru=stru, bz=bz, nao=nao, nabf=nabf, wfc=wfc, cs=cs, vq=V)
 
 
-def push(h, ds: dict, routing: int):
-    """The driver's setter sequence (driver/main.cpp:124-181)."""
+def read_vxc(path):
+    """``vxc_out`` (read_vxc, driver/read_data.cpp:142-182): n_k, n_spin, n_states, then "Ha eV" per
+    state, k outer, spin inner.  Returns [spin][k][state] in Ha."""
+    t = _tokens(path)
+    nk, ns, nst = int(t[0]), int(t[1]), int(t[2])
+    v = np.zeros((ns, nk, nst))
+    p = 3
+    for k in range(nk):
+        for s in range(ns):
+            for i in range(nst):
+                v[s, k, i] = float(t[p])
+                p += 2
+    return v
+
+
+def push(h, ds: dict, routing: int, vq_cut=None):
+    """The driver's setter sequence (driver/main.cpp:124-181); ``vq_cut`` (nk, n, n) is passed as
+    the truncated Coulomb matrix of W (read_Vq_row with is_cut = true)."""
     b, st, bz = ds["band"], ds["stru"], ds["bz"]
     h.set_scf_dimension(b["nspin"], b["nk"], b["nstates"], b["nbasis"])
     h.set_wg_ekb_efermi(b["nspin"], b["nk"], b["nstates"], b["occ"].reshape(-1), b["eps"].reshape(-1),
@@ -242,6 +258,9 @@ def push(h, ds: dict, routing: int):
             for J in range(I, nat):
                 blk = ds["vq"][ik, off[I]:off[I + 1], off[J]:off[J + 1]]
                 h.set_aux_bare_coulomb_k_atom_pair(ik, I, J, blk.shape[0], blk.shape[1], blk.real, blk.imag, 0.0)
+                if vq_cut is not None:
+                    c = vq_cut[ik, off[I]:off[I + 1], off[J]:off[J + 1]]
+                    h.set_aux_cut_coulomb_k_atom_pair(ik, I, J, c.shape[0], c.shape[1], c.real, c.imag, 0.0)
 
 
 def to_system(ds: dict, nfreq: int):

@@ -141,3 +141,45 @@ def test_g0w0_without_cut_coulomb_is_zero():
     o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
     h.build_g0w0_sigma(o)
     assert np.abs(h.get_sigma_block(0, 0, 0, 1, (0, 0, 0), 3, 2)).max() == 0.0
+
+
+@pytest.mark.parametrize("case", ["tiny_k2_mixed", "tiny_gamma"])
+def test_g0w0_truncated_coulomb_differs(case):
+    """v_c != v: W through the square roots of v and v_c (Newton-Schulz on the device)."""
+    _say(f"g0w0 truncated coulomb case={case}")
+    s = CASES[case]()
+    abf, ao = rpa.Basis(s.nabf), rpa.Basis(s.nao)
+    rng = np.random.default_rng(3)
+    n = s.nabf_tot
+    vc = np.empty_like(s.vq)
+    for k in range(s.nk):
+        b = rng.standard_normal((n, n)) * 0.05
+        vc[k] = 0.8 * s.vq[k] + (b @ b.T if np.all(s.kfrac[k] == 0) else b @ b.T + 0j)
+    h = L.Handler()
+    L.load_system(h, s, vq_cut=vc)
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    h.build_g0w0_sigma(o)
+    grid = h.get_tfgrids(o)
+    chi0q = rpa.chi0_q_omega(s, grid, rpa.chi0s_all_tau(s, grid))
+    wrt = gw.w_r_tau(s, grid, gw.wc_q_omega(s, grid, chi0q, vq_cut=vc))
+    Rs = rpa.r_grid(s.period)
+    num = den = 0.0
+    for it in range(s.nfreq):
+        for R in Rs:
+            ref = wrt[(it, R)]
+            for I in range(s.natom):
+                for J in range(s.natom):
+                    got = h.get_wc_block(it, I, J, R, int(s.nabf[I]), int(s.nabf[J]))
+                    num = max(num, np.abs(got - ref[abf.sl(I), abf.sl(J)]).max())
+            den = max(den, np.abs(ref).max())
+    _say(f"  W(R,tau) rel err {num / den:.3e}")
+    assert num / den < 1e-10
+    sig = gw.sigma_all(s, grid, wrt)
+    num = den = 0.0
+    for iw in range(s.nfreq):
+        for R in Rs:
+            ref = sig[(iw, R)]
+            got = h.get_sigma_block(0, iw, 0, 1, R, int(s.nao[0]), int(s.nao[1]))
+            num = max(num, np.abs(got - ref[ao.sl(0), ao.sl(1)]).max())
+            den = max(den, np.abs(ref).max())
+    assert num / den < 1e-10
