@@ -44,7 +44,7 @@ struct GemmProb {
     int kt_off;    // offset (in pairs) into the K table, -1 if none
 };
 
-// Tile configurations (BM x BN, 256 threads).
+// Tile configurations (BM x BN; 256 threads, the w8 shapes 512).
 enum TileCfg : int {
     T64x64 = 0,
     T128x128 = 1,
@@ -53,7 +53,9 @@ enum TileCfg : int {
     T64x128 = 4,
     T16x256 = 5,
     T32x64 = 6,
-    NUM_TILE_CFG = 7
+    T128x128w8 = 7,
+    T128x64w8 = 8,
+    NUM_TILE_CFG = 9
 };
 // Largest square tile that runs well for a precision: 64x64 for FP64 (two waves per SIMD;
 // 128x128 FP64 tiles leave one wave per SIMD and measure ~20% slower), 128x128 otherwise.

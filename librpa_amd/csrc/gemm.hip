@@ -1,7 +1,6 @@
 // Grouped MFMA GEMM kernels for gfx950.
 //
-// One workgroup (4 waves, 256 threads) computes a BM x BN tile of one
-// problem.  Tiles of all problems of a batch form one flat grid; a binary
+// One workgroup (4 or 8 waves) computes a BM x BN tile of one problem.  Tiles of all problems of a batch form one flat grid; a binary
 // search over the per-problem tile prefix finds the problem.  The flat tile
 // index is remapped so that each XCD (8 per MI355X, workgroups dispatched
 // round-robin) receives a contiguous range of tiles, which keeps tiles that
@@ -76,7 +75,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
 // atomic epilogue (kept out of the plain kernels, where they cost registers), 3 plain code
 // under a separate symbol (the chi0 contraction, so that kernel traces can isolate it).
 template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB, int VAR>
-__global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
+__global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
                                                     const typename ModeTraits<MODE>::T *__restrict__ A,
                                                     const typename ModeTraits<MODE>::T *__restrict__ B,
                                                     typename ModeTraits<MODE>::T *__restrict__ C,
@@ -90,8 +89,8 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GemmProb *__restrict__
     constexpr int BK = ModeTraits<MODE>::BK;
     constexpr int PAD = ModeTraits<MODE>::PAD;
     constexpr int LDK = BK + PAD;
-    constexpr int NT = 256;
-    static_assert(WM * WN == 4, "4 waves");
+    constexpr int NT = WM * WN * 64;
+    static_assert(WM * WN == 4 || WM * WN == 8, "4 or 8 waves");
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     static_assert(FM >= 1 && FN >= 1, "wave tile");
@@ -287,7 +286,7 @@ struct TileShape { int bm, bn, wm, wn; const char *name; };
 static const TileShape kTiles[NUM_TILE_CFG] = {
     {64, 64, 2, 2, "64x64"},   {128, 128, 2, 2, "128x128"}, {32, 128, 1, 4, "32x128"},
     {128, 64, 2, 2, "128x64"}, {64, 128, 2, 2, "64x128"},   {16, 256, 1, 4, "16x256"},
-    {32, 64, 2, 2, "32x64"},
+    {32, 64, 2, 2, "32x64"},     {128, 128, 2, 4, "128x128w8"}, {128, 64, 4, 2, "128x64w8"},
 };
 int tile_bm(int cfg) { return kTiles[cfg].bm; }
 int tile_bn(int cfg) { return kTiles[cfg].bn; }
@@ -329,11 +328,14 @@ template <int MODE, int CFG, bool TA, bool TB, int VAR>
 static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C, double alpha, double beta,
                        hipStream_t s) {
     using T = typename ModeTraits<MODE>::T;
-    constexpr int BM = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 32 : (CFG == 3) ? 128 : (CFG == 4) ? 64 : (CFG == 5) ? 16 : 32;
-    constexpr int BN = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 128 : (CFG == 3) ? 64 : (CFG == 4) ? 128 : (CFG == 5) ? 256 : 64;
-    constexpr int WM = (CFG == 2 || CFG == 5) ? 1 : 2;
-    constexpr int WN = 4 / WM;
-    hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB, VAR>), dim3(b.total_tiles), dim3(256), 0, s,
+    constexpr int BM = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 32 : (CFG == 3) ? 128 : (CFG == 4) ? 64
+                     : (CFG == 5) ? 16 : (CFG == 6) ? 32 : 128;
+    constexpr int BN = (CFG == 0) ? 64 : (CFG == 1) ? 128 : (CFG == 2) ? 128 : (CFG == 3) ? 64 : (CFG == 4) ? 128
+                     : (CFG == 5) ? 256 : (CFG == 6) ? 64 : (CFG == 7) ? 128 : 64;
+    // 8-wave tiles: 128x128 as 2 x 4 waves of 64x32, 128x64 as 4 x 2 waves of 32x32
+    constexpr int WM = (CFG == 2 || CFG == 5) ? 1 : (CFG == 8) ? 4 : 2;
+    constexpr int WN = (CFG == 7) ? 4 : (CFG == 8) ? 2 : 4 / WM;
+    hipLaunchKernelGGL((ggemm_kernel<MODE, BM, BN, WM, WN, TA, TB, VAR>), dim3(b.total_tiles), dim3(WM * WN * 64), 0, s,
                        b.d_probs.as<GemmProb>(), (int)b.probs.size(), b.total_tiles, static_cast<const T *>(A),
                        static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(),
                        b.d_coltab.as<long long>(), b.d_ktab.as<long long>(), (T)alpha, (T)beta,
@@ -352,6 +354,8 @@ static void launch_cfg(const GemmBatch &b, const void *A, const void *B, void *C
         case 4: launch_one<MODE, 4, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
         case 5: launch_one<MODE, 5, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
         case 6: launch_one<MODE, 6, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 7: launch_one<MODE, 7, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
+        case 8: launch_one<MODE, 8, TA, TB, VAR>(b, A, B, C, alpha, beta, s); break;
         default: LRA_THROW("bad tile cfg");
     }
 }

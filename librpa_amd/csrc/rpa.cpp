@@ -304,7 +304,8 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         for (int o = c.o_begin; o < c.o_end; ++o) {
             const auto &O = outs[o];
             if (ksum[O.I] == 0) continue;
-            c.fin.add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff, na[O.J],
+            c.fin[(na[O.I] > 64 ? 2 : 0) + (na[O.J] > 64 ? 1 : 0)].add(
+                make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff, na[O.J],
                                 O.eoff, na[O.J]));
         }
         const bool bf = prec == Prec::BF16X3;
@@ -313,15 +314,20 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.xb.trans_a = true;
         c.xb.finalize(T64x64, stream);
         c.ta.finalize(T64x128, stream);
-        c.fin.tag = 3;
-        c.fin.finalize(T64x64, stream);
+        c.fin_flops = 0;
+        for (auto &f : c.fin) {
+            f.tag = 3;
+            f.finalize(T64x64, stream);
+            c.fin_flops += f.flops;
+        }
         c.d_jobs = upload(c.jobs, stream);
         c.d_colsrc = upload(c.colsrc, stream);
-        flops_chi0_exec += c.wb.flops + c.xb.flops + c.ta.flops + c.fin.flops;
+        flops_chi0_exec += c.wb.flops + c.xb.flops + c.ta.flops + c.fin_flops;
         // algorithmic HBM bytes of one contraction launch: Cfin_I and T read once, chi0s read and written
         c.fin_bytes = 0;
-        for (const auto &p : c.fin.probs)
-            c.fin_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + 2.0 * p.M * p.N);
+        for (const auto &f : c.fin)
+            for (const auto &p : f.probs)
+                c.fin_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + 2.0 * p.M * p.N);
         c.ta_bytes = 0;
         for (const auto &p : c.ta.probs)
             c.ta_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + (double)p.M * p.N);
@@ -506,7 +512,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     // the X term below resets it for this tau
                     c.ta.tune(prec, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                     gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                    c.fin.tune(prec, d_Cfin.get(), d_T.get(), chi_base, 1.0, 0.0, stream);
+                    for (auto &f : c.fin) f.tune(prec, d_Cfin.get(), d_T.get(), chi_base, 1.0, 0.0, stream);
                 }
                 prof.begin("chi0_X", stream);
                 gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
@@ -515,8 +521,8 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                 prof.end("chi0_TA", stream, c.ta.flops, c.ta_bytes);
                 prof.begin("chi0_contract", stream);
-                gemm_run(prec, c.fin, d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
-                prof.end("chi0_contract", stream, c.fin.flops, c.fin_bytes);
+                for (auto &f : c.fin) gemm_run(prec, f, d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
+                prof.end("chi0_contract", stream, c.fin_flops, c.fin_bytes);
             }
         }
     }

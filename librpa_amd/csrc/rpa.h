@@ -47,7 +47,10 @@ struct RpaEngine {
         std::vector<long long> colsrc;
         DevBuf d_jobs, d_colsrc;
         int max_gcol = 0;
-        GemmBatch wb, xb, ta, fin;
+        // contraction batches by (na(I) > 64, na(J) > 64), so that each shape class gets its own
+        // tile (a tile covering all of na(I) reads T once)
+        GemmBatch wb, xb, ta, fin[4];
+        double fin_flops = 0;
     };
     std::vector<Chunk> chunks;
     std::vector<long long> w_off;  // W(K, J, S) block of (J, S) in d_W

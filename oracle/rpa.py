@@ -262,6 +262,55 @@ def rpa_ec(sys, grid, chi0q, vq=None):
     return float(contrib.sum().real), contrib
 
 
+def freq_accumulate(sys, grid, chi0s_tau, it, acc):
+    """acc[(I, J, R)][w] += (2/N_s) gamma_c(w, tau_it) chi0s(R, tau_it).
+
+    The cosine half of ``chi_libri_ft_ct`` (src/core/chi0.cpp:507-596) applied one tau at a
+    time, so chi0s of all tau need not be held at once (the R -> q half is ``rpa_ec_streamed``).
+    """
+    col = (2.0 / sys.nspin) * np.asarray(grid['cos_t2f'])[:, it]
+    for key, blk in chi0s_tau.items():
+        add = col[:, None, None] * blk[None, :, :]
+        if key in acc:
+            acc[key] += add
+        else:
+            acc[key] = add.copy()
+    return acc
+
+
+def rpa_ec_streamed(sys, grid, acc, vq=None):
+    """``rpa_ec`` of the chi0(q, w) that ``chi0_q_omega`` would build from the same chi0s.
+
+    ``acc`` is the output of ``freq_accumulate`` over all tau.  Per q the R -> q phase sum
+    e^{i 2 pi q.R} (chi0.cpp:587-593) forms every omega at once; the Hermitian fill, the Gamma
+    real-part rule and the log-det are those of ``rpa_ec``.
+    """
+    abf = Basis(sys.nabf)
+    vq = sys.vq if vq is None else vq
+    gamma_only = sys.nk == 1
+    nq, nw, n = sys.nk, len(grid['omega']), abf.total
+    by_pair: Dict[Tuple[int, int], List] = {}
+    for (I, J, R), a in acc.items():
+        by_pair.setdefault((I, J), []).append((R, a))
+    contrib = np.zeros(nq, dtype=np.complex128)
+    for iq in range(nq):
+        q = sys.kfrac[iq]
+        chi = np.zeros((nw, n, n), dtype=np.complex128)
+        for (I, J), lst in by_pair.items():
+            blk = 0
+            for R, a in lst:
+                blk = blk + np.exp(2j * np.pi * float(np.dot(q, R))) * a
+            chi[:, abf.sl(I), abf.sl(J)] = blk
+            if J != I:
+                chi[:, abf.sl(J), abf.sl(I)] = np.conj(np.transpose(blk, (0, 2, 1)))
+        V = vq[iq]
+        for iw in range(nw):
+            Pi = V.real @ chi[iw].real if gamma_only else V @ chi[iw]
+            val = np.trace(Pi) + ln_det_ref(np.eye(n) - Pi)
+            contrib[iq] += complex(val) * grid['omega_w'][iw] * (1.0 / nq) / (2 * np.pi)
+    return float(contrib.sum().real), contrib
+
+
 def vq_upper_blocks(sys, ik):
     """Upper (I <= J) atom-pair blocks of V(k) as the API receives them."""
     abf = Basis(sys.nabf)

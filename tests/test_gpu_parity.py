@@ -19,7 +19,7 @@ def _say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("tile", range(7))
+@pytest.mark.parametrize("tile", range(9))
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("prec,tol", [(L.FP64, 1e-13), (L.FP32, 3e-6), (L.BF16X3, 3e-5)])
 def test_gemm_engine(prec, tol, ta, tb, tile, lib):

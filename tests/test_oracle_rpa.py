@@ -101,8 +101,45 @@ def test_transform_matches_explicit_sum():
     assert np.abs(q[(iw, iq, I, J)] - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("kw", [
+    dict(natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0, nfreq=6),
+    dict(natom=3, period=(1, 1, 1), nao=(2, 3), nabf=(3, 4), nocc=(1, 1), a=6.5, seed=11, nfreq=6),
+])
+def test_streamed_ec_equals_loop_form(kw):
+    # the per-tau accumulation + per-q assembly used for large goldens must give the same
+    # Ec and per-k contributions as chi0_q_omega + rpa_ec (k-point and Gamma-only systems)
+    s = _sys(**kw)
+    r = rpa.rpa_from_system(s)
+    acc = {}
+    for it, c in enumerate(r["chi0s"]):
+        rpa.freq_accumulate(s, r["grid"], c, it, acc)
+    ec, contrib = rpa.rpa_ec_streamed(s, r["grid"], acc)
+    assert abs(ec - r["ec"]) < 1e-13 * abs(r["ec"])
+    assert np.abs(contrib - r["contrib"]).max() < 1e-13 * abs(r["ec"])
+
+
 def test_cell2_ec_negative_and_sums():
     s = _sys(natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0, nfreq=6)
     r = rpa.rpa_from_system(s)
     assert r["ec"] < 0
     assert abs(r["contrib"].sum().real - r["ec"]) < 1e-12 * abs(r["ec"])
+
+
+def test_lu_logdet_imaginary_part_is_pivot_sign_artefact():
+    # compute_pi_det_blacs_2d (epsilon.cpp:623-645) sums log(+-d_i) over the LU pivots,
+    # flipping the sign of pivots with negative real part.  For 1 - V chi0 (det real and
+    # positive) that leaves Im ln det = m pi with integer m set by the pivot order, and
+    # Re ln det = ln det.  The GPU path reports the principal value (Im = 0).
+    s = _sys(natom=2, period=(2, 2, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0, nfreq=6)
+    r = rpa.rpa_from_system(s)
+    abf = rpa.Basis(s.nabf)
+    for iq in range(s.nk):
+        for iw in range(6):
+            chi = rpa.full_matrix(r["chi0q"], abf, s.natom, lambda I, J: r["chi0q"][(iw, iq, I, J)])
+            M = np.eye(abf.total) - s.vq[iq] @ chi
+            ld = rpa.ln_det_ref(M)
+            sign, logabs = np.linalg.slogdet(M)
+            assert abs(sign.imag) < 1e-10 and sign.real > 0
+            assert abs(ld.real - logabs) < 1e-10
+            m = ld.imag / np.pi
+            assert abs(m - round(m)) < 1e-8
