@@ -75,9 +75,9 @@ struct GemmBatch {
     bool trans_b = false;  // B stored as B^T (element (k,n) at b_off + n*ldb + k)
     bool lower_only = false;
     bool atomic = false;   // C += alpha A B by atomic adds (beta must be 1)
-    int tag = 0;           // 3: launch under the separately named (traceable) plain instantiation
+    int tag = 0;           // 3 / 4: launch under a separately named (traceable) plain instantiation
     int total_tiles = 0;
-    double flops = 0;      // algorithmic flops of one launch (2 M N K summed)
+    double flops = 0;      // flops of one launch: 2 M N K summed, less the skipped triangular work
     DevBuf d_probs, d_rowtab, d_coltab, d_ktab;
 
     bool tuned = false;

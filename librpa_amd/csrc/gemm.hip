@@ -72,8 +72,9 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
 }
 
 // VAR selects the addressing extras compiled in: 0 plain, 1 K table, 2 column table and
-// atomic epilogue (kept out of the plain kernels, where they cost registers), 3 plain code
-// under a separate symbol (the chi0 contraction, so that kernel traces can isolate it).
+// atomic epilogue (kept out of the plain kernels, where they cost registers), 3 and 4 plain
+// code under separate symbols (the chi0 contraction and T_A, so that kernel traces can
+// isolate them).
 template <int MODE, int BM, int BN, int WM, int WN, bool TA, bool TB, int VAR>
 __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__restrict__ probs, int nprob, int total,
                                                     const typename ModeTraits<MODE>::T *__restrict__ A,
@@ -314,7 +315,17 @@ void GemmBatch::finalize(int cfg_, hipStream_t s) {
         p.tiles_n = ceil_div(p.N, bn);
         p.tile0 = t0;
         t0 += ceil_div(p.M, bm) * p.tiles_n;
-        flops += 2.0 * p.M * p.N * (double)p.K;
+        if (p.kmode == 0 && !lower_only) {
+            flops += 2.0 * p.M * p.N * (double)p.K;
+        } else {
+            // executed work: tiles above the diagonal skipped, K started at the tile's diagonal
+            for (int m0 = 0; m0 < p.M; m0 += bm)
+                for (int n0 = 0; n0 < p.N; n0 += bn) {
+                    if (lower_only && n0 > m0 + bm - 1) continue;
+                    const int kb = p.kmode == 1 ? n0 : (p.kmode == 2 ? m0 : 0);
+                    flops += 2.0 * std::min(bm, p.M - m0) * std::min(bn, p.N - n0) * (double)std::max(0, p.K - kb);
+                }
+        }
     }
     total_tiles = t0;
     d_probs = upload(probs, s);
@@ -378,10 +389,11 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
         else launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
         return;
     }
-    if (b.tag == 3) {
+    if (b.tag == 3 || b.tag == 4) {
         // same code as the plain kernel, instantiated separately so that traces can tell it apart
         if (b.trans_a || b.trans_b) LRA_THROW("tagged GEMM: plain A and B");
-        launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
+        if (b.tag == 3) launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
+        else launch_cfg<MODE, false, false, 4>(b, A, B, C, alpha, beta, s);
         return;
     }
     if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);

@@ -313,6 +313,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.wb.finalize(big_tile(prec), stream);
         c.xb.trans_a = true;
         c.xb.finalize(T64x64, stream);
+        c.ta.tag = 4;
         c.ta.finalize(T64x128, stream);
         c.fin_flops = 0;
         for (auto &f : c.fin) {
