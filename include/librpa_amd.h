@@ -65,8 +65,38 @@ int librpa_amd_get_sigma_block(LibrpaHandler *h, int ispin, int ifreq, int I, in
  * k point ik for all nfreq frequencies (ascending).  Returns nfreq, < 0 on error. */
 int librpa_amd_get_sigc_ks(LibrpaHandler *h, int ispin, int ik, int i_state, double *re, double *im);
 
+/* Block-norm screening of the chi0 build (nspin = 1).  mode 0: off (default);
+ * 1: skip output blocks (I, J, R) at tau whose norm-product bound
+ * c_I c_J (g+_IJ gm- + g-_IJ gm+ + 2 gm+ gm-) is below threshold; 2: also skip
+ * blocks whose learned prediction of ||chi0s(I,J,R; tau)||_F is below threshold
+ * (model: LIBRPA_AMD_SCREEN_MODEL floats from librpa_amd_screen_train).  Skipped
+ * blocks are zero in chi0(R, tau). */
+#define LIBRPA_AMD_SCREEN_NFEAT 8
+#define LIBRPA_AMD_SCREEN_NPARAM 4801
+#define LIBRPA_AMD_SCREEN_MODEL (LIBRPA_AMD_SCREEN_NPARAM + 2 * LIBRPA_AMD_SCREEN_NFEAT + 2)
+void librpa_amd_set_screening(LibrpaHandler *h, int mode, double threshold, const float *model);
+/* Screening counters of the last chi0 build: [0] blocks x tau considered, [1] skipped,
+ * [2] GEMM flops of the screened batches, [3] flops skipped.  Returns 4. */
+int librpa_amd_get_screen_stats(LibrpaHandler *h, double *out);
+/* Training samples of the last chi0 build made with librpa_amd_set_keep_chi0(h, 1) and
+ * screening off: per (tau, output block) the 8 features (distance / 10 bohr, n_b(I) / 27,
+ * n_b(J) / 27, ln tau, ln c_I, ln c_J, ln(g+_IJ g-_IJ), ln(gm+ gm-)) into X [n][8] and
+ * ln ||chi0s||_F into t [n]; blocks with zero norm are left out.  NULL pointers return the
+ * count only.  Returns n. */
+int librpa_amd_get_screen_samples(LibrpaHandler *h, float *X, float *t);
+/* Trains the screening MLP (8 -> 64 -> 64 -> 1, SiLU, FP32) on the device: standardises
+ * X and t, runs iters full-batch Adam steps (learning rate lr decaying to lr / 20) from a
+ * seeded initialisation, and writes the model.  Returns the final mean squared error in
+ * standardised units (NaN on error). */
+double librpa_amd_screen_train(const float *X, const float *t, int n, int iters, double lr, int seed, float *model);
+/* MLP on standardised inputs, for verification: forward pass y [n], and the mean squared
+ * error against t with its gradient in the parameter layout W1[8][64], b1[64], W2[64][64],
+ * b2[64], w3[64], b3.  Return 0 on success. */
+int librpa_amd_mlp_predict(const float *params, const float *X, int n, float *y);
+int librpa_amd_mlp_loss_grad(const float *params, const float *X, const float *t, int n, float *grad, double *loss);
+
 /* One dense GEMM C = alpha op(A) op(B) + beta C on host arrays through the
- * device engine (prec as above; tile_cfg 0..6 selects the workgroup tile).
+ * device engine (prec as above; tile_cfg 0..8 selects the workgroup tile).
  * For engine verification.  Returns 0 on success. */
 int librpa_amd_gemm(int prec, int trans_a, int trans_b, int M, int N, int K, double alpha, const double *A, int lda,
                     const double *B, int ldb, double beta, double *C, int ldc, int tile_cfg);

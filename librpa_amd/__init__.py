@@ -102,6 +102,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _hp = C.POINTER(_Handler)
 _op = C.POINTER(Options)
+_fp = C.POINTER(C.c_float)
 
 _SIGS = {
     "librpa_get_build_info": (C.c_char_p, []),
@@ -156,7 +157,19 @@ _SIGS = {
     "librpa_amd_get_stats": (C.c_int, [_hp, _dp, C.c_int]),
     "librpa_amd_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
                                   _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_int]),
+    "librpa_amd_set_screening": (None, [_hp, C.c_int, C.c_double, _fp]),
+    "librpa_amd_get_screen_stats": (C.c_int, [_hp, _dp]),
+    "librpa_amd_get_screen_samples": (C.c_int, [_hp, _fp, _fp]),
+    "librpa_amd_screen_train": (C.c_double, [_fp, _fp, C.c_int, C.c_int, C.c_double, C.c_int, _fp]),
+    "librpa_amd_mlp_predict": (C.c_int, [_fp, _fp, C.c_int, _fp]),
+    "librpa_amd_mlp_loss_grad": (C.c_int, [_fp, _fp, _fp, C.c_int, _fp, _dp]),
 }
+
+# block-norm screener (librpa_amd.h): MLP sizes
+SCREEN_NFEAT = 8
+SCREEN_NPARAM = 4801
+SCREEN_MODEL = SCREEN_NPARAM + 2 * SCREEN_NFEAT + 2
+SCREEN_OFF, SCREEN_BOUND, SCREEN_LEARNED = 0, 1, 2
 
 
 def load(path: Optional[str] = None):
@@ -186,6 +199,41 @@ def _check():
 def _d(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def screen_train(X, t, iters=2000, lr=3e-3, seed=0):
+    """Train the screening MLP on the device; returns (model, final standardised MSE)."""
+    X, xp = _f(X)
+    t, tp = _f(t)
+    model = np.zeros(SCREEN_MODEL, dtype=np.float32)
+    loss = load().librpa_amd_screen_train(xp, tp, len(t), iters, lr, seed, model.ctypes.data_as(_fp))
+    _check()
+    return model, loss
+
+
+def mlp_predict(params, X):
+    p, pp = _f(params)
+    X, xp = _f(X)
+    y = np.zeros(len(X), dtype=np.float32)
+    load().librpa_amd_mlp_predict(pp, xp, len(X), y.ctypes.data_as(_fp))
+    _check()
+    return y
+
+
+def mlp_loss_grad(params, X, t):
+    p, pp = _f(params)
+    X, xp = _f(X)
+    t, tp = _f(t)
+    g = np.zeros(SCREEN_NPARAM, dtype=np.float32)
+    loss = np.zeros(1)
+    load().librpa_amd_mlp_loss_grad(pp, xp, tp, len(t), g.ctypes.data_as(_fp), loss.ctypes.data_as(_dp))
+    _check()
+    return float(loss[0]), g
 
 
 def init_global():
@@ -381,6 +429,22 @@ class Handler:
         self._call("librpa_amd_get_tfgrids", C.byref(opts), *[a.ctypes.data_as(_dp) for a in arrs])
         keys = ["tau", "tau_w", "omega", "omega_w", "cos_t2f", "cos_f2t", "sin_t2f"]
         return dict(zip(keys, arrs))
+
+    def set_screening(self, mode=SCREEN_OFF, threshold=1e-8, model=None):
+        m = None if model is None else np.ascontiguousarray(model, dtype=np.float32)
+        self._call("librpa_amd_set_screening", mode, threshold, None if m is None else m.ctypes.data_as(_fp))
+
+    def get_screen_stats(self):
+        out = np.zeros(4)
+        self._call("librpa_amd_get_screen_stats", out.ctypes.data_as(_dp))
+        return dict(zip(["blocks", "skipped", "flops", "flops_skipped"], out.tolist()))
+
+    def get_screen_samples(self):
+        n = self._call("librpa_amd_get_screen_samples", None, None)
+        X = np.zeros((n, SCREEN_NFEAT), dtype=np.float32)
+        t = np.zeros(n, dtype=np.float32)
+        self._call("librpa_amd_get_screen_samples", X.ctypes.data_as(_fp), t.ctypes.data_as(_fp))
+        return X, t
 
     def get_stats(self):
         out = np.zeros(8)

@@ -506,6 +506,10 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
                 eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
             }
             eng->ec_prec = d.ec_prec;
+            eng->screen_mode = d.screen_mode;
+            eng->screen_thr = d.screen_thr;
+            eng->screen_model = d.screen_model;
+            eng->collect = d.keep_chi0 && d.screen_mode == 0;
             eng->build_chi0(d, d.grid, o.n_bands_chi0);
             auto r = eng->ec(d, d.grid);
             if ((int)r.qcontrib.size() != n_ibz_kpoints && o.output_level >= LIBRPA_VERBOSE_WARN)
@@ -582,6 +586,86 @@ int librpa_amd_get_last_error(char *buf, int len) {
     }
     g_last_error.clear();
     return 1;
+}
+
+void librpa_amd_set_screening(LibrpaHandler *h, int mode, double threshold, const float *model) {
+    guarded_void([&]() {
+        auto &d = ds_of(h);
+        if (mode < 0 || mode > 2) LRA_THROW("screening mode must be 0, 1 or 2");
+        if (mode > 0 && !(threshold > 0)) LRA_THROW("screening threshold must be positive");
+        if (mode == 2 && !model) LRA_THROW("learned screening needs a model");
+        d.screen_mode = mode;
+        d.screen_thr = threshold;
+        if (mode == 2) d.screen_model.assign(model, model + MLP_NMODEL);
+        else d.screen_model.clear();
+    });
+}
+
+int librpa_amd_get_screen_stats(LibrpaHandler *h, double *out) {
+    return guarded(
+        [&]() -> int {
+            auto &d = ds_of(h);
+            if (!d.rpa) LRA_THROW("no chi0 build yet");
+            out[0] = d.rpa->scr_blocks;
+            out[1] = d.rpa->scr_skipped;
+            out[2] = d.rpa->scr_flops;
+            out[3] = d.rpa->scr_flops_skipped;
+            return 4;
+        },
+        -1);
+}
+
+int librpa_amd_get_screen_samples(LibrpaHandler *h, float *X, float *t) {
+    return guarded(
+        [&]() -> int {
+            auto &d = ds_of(h);
+            if (!d.rpa || d.rpa->features.empty())
+                LRA_THROW("no samples: build chi0 with librpa_amd_set_keep_chi0(h, 1) and screening off");
+            auto &e = *d.rpa;
+            std::vector<double> nrm;
+            e.chi0_norms(nrm);
+            const size_t nout = e.outs.size();
+            int n = 0;
+            for (size_t i = 0; i < nrm.size(); ++i) {
+                if (!(nrm[i] > 0)) continue;
+                if (X) std::memcpy(X + (size_t)n * MLP_NFEAT, &e.features[i * MLP_NFEAT], sizeof(float) * MLP_NFEAT);
+                if (t) t[n] = (float)std::log(nrm[i]);
+                ++n;
+            }
+            (void)nout;
+            return n;
+        },
+        -1);
+}
+
+double librpa_amd_screen_train(const float *X, const float *t, int n, int iters, double lr, int seed, float *model) {
+    return guarded([&]() -> double { return mlp_train(X, t, n, iters, lr, seed, model); }, std::nan(""));
+}
+
+int librpa_amd_mlp_predict(const float *params, const float *X, int n, float *y) {
+    return guarded(
+        [&]() -> int {
+            std::vector<float> p(params, params + MLP_NPARAM), x(X, X + (size_t)n * MLP_NFEAT);
+            DevBuf dp = upload(p), dx = upload(x), dy(sizeof(float) * std::max(1, n));
+            launch_mlp_forward(dp.as<float>(), dx.as<float>(), n, dy.as<float>(), 0);
+            HIP_CHECK(hipMemcpy(y, dy.get(), sizeof(float) * n, hipMemcpyDeviceToHost));
+            return 0;
+        },
+        1);
+}
+
+int librpa_amd_mlp_loss_grad(const float *params, const float *X, const float *t, int n, float *grad, double *loss) {
+    return guarded(
+        [&]() -> int {
+            std::vector<float> p(params, params + MLP_NPARAM), x(X, X + (size_t)n * MLP_NFEAT), tt(t, t + n);
+            DevBuf dp = upload(p), dx = upload(x), dt = upload(tt), dg(sizeof(float) * MLP_NPARAM), dl(sizeof(double));
+            MlpWork w;
+            mlp_loss_grad(dp.as<float>(), dx.as<float>(), dt.as<float>(), n, dg.as<float>(), dl.as<double>(), w, 0);
+            HIP_CHECK(hipMemcpy(grad, dg.get(), sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(loss, dl.get(), sizeof(double), hipMemcpyDeviceToHost));
+            return 0;
+        },
+        1);
 }
 
 void librpa_amd_set_keep_chi0(LibrpaHandler *h, int keep) {

@@ -67,6 +67,9 @@ struct Dataset {
     Prec chi0_prec = Prec::F64;
     Prec ec_prec = Prec::F64;
     bool keep_chi0 = false;
+    int screen_mode = 0;
+    double screen_thr = 1e-8;
+    std::vector<float> screen_model;
 
     // results of the last compute calls
     TFGrid grid;

@@ -75,6 +75,11 @@ struct GemmBatch {
     bool trans_b = false;  // B stored as B^T (element (k,n) at b_off + n*ldb + k)
     bool lower_only = false;
     bool atomic = false;   // C += alpha A B by atomic adds (beta must be 1)
+    // per-problem skip flags (screening): skipped problems return, or with skip_zero write
+    // alpha * 0 + beta * C; d_skip is filled by the owner before a launch with use_skip set
+    bool use_skip = false, skip_zero = false;
+    std::vector<unsigned char> skipv;
+    DevBuf d_skip;
     int tag = 0;           // 3 / 4: launch under a separately named (traceable) plain instantiation
     int total_tiles = 0;
     double flops = 0;      // flops of one launch: 2 M N K summed, less the skipped triangular work

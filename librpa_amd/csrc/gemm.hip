@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace lra {
 
@@ -84,7 +86,8 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
                                                     const long long *__restrict__ coltab,
                                                     const long long *__restrict__ ktab,
                                                     typename ModeTraits<MODE>::T alpha,
-                                                    typename ModeTraits<MODE>::T beta, int lower_only, int atomic) {
+                                                    typename ModeTraits<MODE>::T beta, int lower_only, int atomic,
+                                                    const unsigned char *__restrict__ skip, int skip_zero) {
     using T = typename ModeTraits<MODE>::T;
     using V = typename ModeTraits<MODE>::V;
     constexpr int BK = ModeTraits<MODE>::BK;
@@ -105,6 +108,9 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
     if (tile >= total) return;
     const int pi = find_prob(probs, nprob, tile);
     const GemmProb P = probs[pi];
+    // a screened-out problem: return, or (skip_zero) write alpha * 0 + beta * C without the K loop
+    const bool dead = skip != nullptr && skip[pi] != 0;
+    if (dead && !skip_zero) return;
     const int lt = tile - P.tile0;
     const int tm = lt / P.tiles_n, tn = lt % P.tiles_n;
     const int m0 = tm * BM, n0 = tn * BN;
@@ -187,6 +193,7 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
     int kbeg = 0;
     if (P.kmode == 1) kbeg = (n0 / BK) * BK;
     else if (P.kmode == 2) kbeg = (m0 / BK) * BK;
+    if (dead) kbeg = P.K;
     gload(kbeg);
     for (int k0 = kbeg; k0 < P.K; k0 += BK) {
         __syncthreads();
@@ -350,7 +357,8 @@ static void launch_one(const GemmBatch &b, const void *A, const void *B, void *C
                        b.d_probs.as<GemmProb>(), (int)b.probs.size(), b.total_tiles, static_cast<const T *>(A),
                        static_cast<const T *>(B), static_cast<T *>(C), b.d_rowtab.as<long long>(),
                        b.d_coltab.as<long long>(), b.d_ktab.as<long long>(), (T)alpha, (T)beta,
-                       (int)b.lower_only, (int)b.atomic);
+                       (int)b.lower_only, (int)b.atomic, b.use_skip ? b.d_skip.as<unsigned char>() : nullptr,
+                       (int)b.skip_zero);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -402,9 +410,49 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
     else launch_cfg<MODE, false, false>(b, A, B, C, alpha, beta, s);
 }
 
+namespace {
+// tile choices of batches already tuned in this process, keyed by the batch's shape signature
+// (precision, variant, transposes, and every problem's M, N, K, strides and addressing mode), so
+// that a second handler with the same inputs does not time the candidates again
+std::mutex g_tune_mu;
+std::map<std::vector<long long>, std::pair<int, float>> &tune_cache() {
+    static std::map<std::vector<long long>, std::pair<int, float>> c;
+    return c;
+}
+std::vector<long long> tune_key(Prec prec, const GemmBatch &b) {
+    std::vector<long long> k = {(long long)prec, b.tag, b.trans_a, b.trans_b, b.lower_only,
+                                (long long)b.rowtab.size(), (long long)b.coltab.size(), (long long)b.ktab.size()};
+    k.reserve(k.size() + 8 * b.probs.size());
+    for (const auto &p : b.probs) {
+        k.push_back(p.M);
+        k.push_back(p.N);
+        k.push_back(p.K);
+        k.push_back(p.lda);
+        k.push_back(p.ldb);
+        k.push_back(p.ldc);
+        k.push_back(p.kmode + 4 * (p.rt_off >= 0) + 8 * (p.ct_off >= 0) + 16 * (p.kt_off >= 0));
+        k.push_back(p.ncb < p.N ? p.ncb : 0);
+    }
+    return k;
+}
+}  // namespace
+
 int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alpha, double beta, hipStream_t s,
                     const std::function<void()> &before) {
     if (probs.empty() || atomic) return cfg;
+    const auto key = tune_key(prec, *this);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+            finalize(it->second.first, s);
+            tuned = true;
+            tuned_ms = it->second.second;
+            return cfg;
+        }
+    }
+    const bool was_skip = use_skip;
+    use_skip = false;
     std::vector<int> cands;
     if (!ktab.empty()) cands = {T64x64};
     else if (!coltab.empty()) cands = {T64x64, T128x128};
@@ -412,12 +460,7 @@ int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alp
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    float best = 1e30f;
-    int bc = cfg;
-    for (int c : cands) {
-        finalize(c, s);
-        if (before) before();
-        gemm_run(prec, *this, A, B, C, alpha, beta, s);
+    auto timed = [&](void) {
         if (before) before();
         HIP_CHECK(hipEventRecord(e0, s));
         gemm_run(prec, *this, A, B, C, alpha, beta, s);
@@ -425,13 +468,29 @@ int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alp
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    float best = 1e30f;
+    int bc = cfg;
+    for (int c : cands) {
+        finalize(c, s);
+        // first run warms the code object; a candidate already 1.5x slower than the best
+        // is not timed again
+        const float first = timed();
+        if (first > 1.5f * best) continue;
+        const float ms = std::min(first, timed());
         if (ms < best) { best = ms; bc = c; }
     }
     HIP_CHECK(hipEventDestroy(e0));
     HIP_CHECK(hipEventDestroy(e1));
     finalize(bc, s);
+    use_skip = was_skip;
     tuned = true;
     tuned_ms = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_cache()[key] = {bc, best};
+    }
     return bc;
 }
 

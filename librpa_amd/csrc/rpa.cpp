@@ -109,10 +109,14 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     // packed Cs in the two GEMM layouts
     {
         std::vector<double> cfin(cfin_off[natom]), cwb(cwb_off[natom]);
+        cnorm.assign(natom, 0.0);
         for (int I = 0; I < natom; ++I) {
             for (size_t n = 0; n < nbrs[I].size(); ++n) {
                 const int K = nbrs[I][n].K;
                 const auto &blk = folded.at({I, K, nbrs[I][n].r});
+                double ss = 0;
+                for (double v : blk) ss += v * v;
+                cnorm[I] += std::sqrt(ss);
                 const int ni = nb[I], nkk = nb[K], nm = na[I];
                 for (int i = 0; i < ni; ++i)
                     for (int k = 0; k < nkk; ++k)
@@ -268,6 +272,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                                     w_at(O.J, O.r) + ao[O.I] * na[O.J], 0, O.eoff, na[O.J]);
             pr.kt_off = it->second;
             c.xb.add(pr);
+            c.x_owner.push_back(o);
         }
         // T_A: per (J, S, K, class of I), rows (I, n, i) from G_IJ(S + R1), B = W(K, J, S)
         for (int J = c.j_begin; J < c.j_end; ++J)
@@ -280,6 +285,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                         pr.cbs = (long long)cls * na[J];
                         pr.rt_off = (int)(c.ta.rowtab.size() / 2);
                         int m = 0;
+                        const size_t owners0 = c.ta_outs.size();
                         const auto &Sv = cells[S];
                         for (const auto &in : rev[K]) {
                             const int I = in.first, n = in.second;
@@ -287,6 +293,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                             const auto &R1 = cells[nbrs[I][n].r];
                             const int rr = fold_index(Sv[0] + R1[0], Sv[1] + R1[1], Sv[2] + R1[2]);
                             const int o = out_index[((size_t)I * natom + J) * nR + rr];
+                            if (c.ta_outs.size() == owners0 || c.ta_outs.back() != o) c.ta_outs.push_back(o);
                             for (int i = 0; i < nb[I]; ++i) {
                                 c.ta.rowtab.push_back(rr * gpm_r + (ao[I] + i) * 2LL * NAO + 2 * ao[J]);
                                 c.ta.rowtab.push_back(outs[o].toff + (koff[I][n] + i) * na[J]);
@@ -295,18 +302,22 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                         }
                         if (m == 0) {
                             c.ta.rowtab.resize((size_t)pr.rt_off * 2);
+                            c.ta_outs.resize(owners0);
                             continue;
                         }
                         pr.M = m;
                         c.ta.add(pr);
+                        if (c.ta_ptr.empty()) c.ta_ptr.push_back(0);
+                        c.ta_ptr.push_back((int)c.ta_outs.size());
                     }
         // final: chi0s(o) = Cfin_I (na_I x ksum_I) * T_o (ksum_I x na_J)
         for (int o = c.o_begin; o < c.o_end; ++o) {
             const auto &O = outs[o];
             if (ksum[O.I] == 0) continue;
-            c.fin[(na[O.I] > 64 ? 2 : 0) + (na[O.J] > 64 ? 1 : 0)].add(
-                make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff, na[O.J],
-                                O.eoff, na[O.J]));
+            const int cls = (na[O.I] > 64 ? 2 : 0) + (na[O.J] > 64 ? 1 : 0);
+            c.fin[cls].add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff,
+                                     na[O.J], O.eoff, na[O.J]));
+            c.fin_owner[cls].push_back(o);
         }
         const bool bf = prec == Prec::BF16X3;
         (void)bf;
@@ -344,6 +355,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     d_Ghat.alloc(max_g * ts);
     d_W.alloc((size_t)w_off.back() * ts);
     d_T.alloc(max_t * ts);
+    screen_plan(ds);
 
     // ---- G build data ----
     {
@@ -487,10 +499,22 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     d_W.reserve((size_t)w_off.back() * ts);
     d_T.reserve(t_max * ts);
     HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
+    if (screen_mode != 0 && nspin > 1) LRA_THROW("chi0 block screening needs nspin = 1");
+    if (screen_mode == 2) {
+        if (screen_model.size() != (size_t)MLP_NMODEL) LRA_THROW("learned screening without a model");
+        d_model = upload(screen_model, stream);
+    }
+    scr_blocks = scr_skipped = scr_flops = scr_flops_skipped = 0;
+    if (collect) features.assign((size_t)ntau * MLP_NFEAT * outs.size(), 0.0f);
+    for (auto &c : chunks) {
+        c.xb.use_skip = c.ta.use_skip = false;
+        for (auto &f : c.fin) f.use_skip = false;
+    }
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
         for (int s = 0; s < nspin; ++s) {
             build_gf(mf, tau, s);
+            if (s == 0 && (screen_mode != 0 || collect)) screen_tau(it, tau);
             char *chi_base = d_chi0s.as<char>() + (size_t)it * E_tot * ts;
             const bool tune_now = autotune && it == 0 && s == 0;
             for (auto &c : chunks) {

@@ -9,6 +9,7 @@
 #include "dataset.h"
 #include "gemm.h"
 #include "kernels.h"
+#include "screen.h"
 
 namespace lra {
 
@@ -51,6 +52,8 @@ struct RpaEngine {
         // tile (a tile covering all of na(I) reads T once)
         GemmBatch wb, xb, ta, fin[4];
         double fin_flops = 0;
+        // screening maps: output of each X / contraction problem, outputs written by each T_A problem
+        std::vector<int> x_owner, fin_owner[4], ta_ptr, ta_outs;
     };
     std::vector<Chunk> chunks;
     std::vector<long long> w_off;  // W(K, J, S) block of (J, S) in d_W
@@ -80,6 +83,24 @@ struct RpaEngine {
     int planned_ntau = 0;
     bool autotune = true;  // pick each batch's tile shape by timing on first use
     std::map<std::tuple<int, int, int>, std::pair<GemmBatch, GemmBatch>> ec_batches;
+
+    // ---- block-norm screening of chi0 outputs (nspin = 1) ----
+    int screen_mode = 0;          // 0 off, 1 norm-product bound, 2 learned MLP and the bound
+    double screen_thr = 1e-8;     // skip blocks whose bound (or prediction) is below this norm
+    std::vector<float> screen_model;  // MLP_NMODEL values (parameters, feature and target scaling)
+    bool collect = false;         // record the features of every (tau, output) of a build
+    std::vector<double> cnorm;    // per atom: sum of the Frobenius norms of its folded Cs blocks
+    std::vector<float> feat_static;  // per output: d / 10, n_b(I) / 27, n_b(J) / 27, ln c_I, ln c_J
+    std::vector<BlockRef> gblocks;   // per output and sign: the G_IJ(R) block inside Gpm
+    DevBuf d_gblocks, d_gnorm, d_feat, d_pred, d_model, d_eoff, d_esize;
+    std::vector<float> features;     // [tau][output][8] of the last build (collect)
+    std::vector<double> gnorm_h;
+    std::vector<unsigned char> skip_out;
+    std::vector<float> pred_h, feat_h;
+    double scr_blocks = 0, scr_skipped = 0, scr_flops = 0, scr_flops_skipped = 0;
+    void screen_plan(const Dataset &ds);
+    void screen_tau(int it, double tau);            // features and skip flags after build_gf
+    void chi0_norms(std::vector<double> &out);      // ||chi0s(o, tau)||_F, [tau][output]
 
     void plan(const Dataset &ds, Prec p, Prec pe, int ntau);
     // G(R, +-tau) of spin s into d_Gpm (prepare_gf once per band cut first)
