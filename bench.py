@@ -43,6 +43,36 @@ def _sync():
         pass
 
 
+def screening_report(L, h, o, s, ec64, thr=1e-8):
+    """Block-norm screening of the FP64 build at threshold thr: the MLP is trained on the block
+    norms of this workload's unscreened FP64 build, then the bound-only and the learned screener
+    each run one evaluation; dEc is against the unscreened FP64 Ec."""
+    h.set_precision(L.FP64, L.FP64)
+    h.set_screening(L.SCREEN_OFF)
+    h.set_keep_chi0(True)
+    h.get_rpa_correlation_energy(o, s.nk)
+    X, t = h.get_screen_samples()
+    h.set_keep_chi0(False)
+    t0 = time.perf_counter()
+    model, loss = L.screen_train(X, t, iters=1500, lr=3e-3, seed=0)
+    out = {"threshold": thr, "trained_on": "this workload's unscreened FP64 build", "train_samples": int(len(t)),
+           "train_mse_standardised": round(float(loss), 6), "train_s": round(time.perf_counter() - t0, 2)}
+    for name, mode in (("bound", L.SCREEN_BOUND), ("learned", L.SCREEN_LEARNED)):
+        h.set_screening(mode, thr, model if mode == L.SCREEN_LEARNED else None)
+        _sync()
+        t0 = time.perf_counter()
+        ec, _ = h.get_rpa_correlation_energy(o, s.nk)
+        _sync()
+        dt = time.perf_counter() - t0
+        st = h.get_screen_stats()
+        out[name] = {"skipped_block_fraction": round(st["skipped"] / max(1.0, st["blocks"]), 4),
+                     "skipped_flop_fraction": round(st["flops_skipped"] / max(1.0, st["flops"]), 4),
+                     "dEc_per_atom_ha": (ec - ec64) / s.natom, "s_per_freq": round(dt / s.nfreq, 5)}
+        _log(f"screening {name}: {out[name]}")
+    h.set_screening(L.SCREEN_OFF)
+    return out
+
+
 def run_steps(L, h, o, nk, prec, steps, warmup):
     h.set_precision(prec, prec)
     ec = None
@@ -186,6 +216,7 @@ def main():
     ap.add_argument("--workload", default="cubic128")
     ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-screen", action="store_true", help="skip the block-norm screening report")
     ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
                     help="where the synthetic Coulomb matrix product is formed (cpu under profilers)")
     ap.add_argument("--task", default="rpa", choices=["rpa", "g0w0"],
@@ -286,6 +317,8 @@ def main():
                              round(v[3] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else 0.0]
                          for k, v in prof.items()},
     }
+    if not args.no_screen:
+        res["screening"] = screening_report(L, h, o, s, ec64)
     if not args.no_cpu_baseline:
         t0 = time.time()
         res["cpu_baseline"] = cpu_baseline(s, grid, s.nfreq)
