@@ -278,6 +278,28 @@ def tiny(seed=7, natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1)
                         real_gamma_v=real_gamma_v)
 
 
+def spin_polarised(s: System, seed: int = 0, shift: float = 0.03) -> System:
+    """The same system with two spin channels: spin 0 keeps the bands of ``s``; spin 1 has band
+    energies moved by U(-shift, shift) (re-sorted, the gap kept) and new random unitary
+    eigenvectors.  Occupations are 1 per occupied state and spin (the API's range for nspin = 2)."""
+    rng = np.random.default_rng(seed + 77)
+    nocc = int(round(s.occ[0, 0].sum() / 2.0))
+    nk, nb = s.eps.shape[1], s.eps.shape[2]
+    eps1 = np.sort(s.eps[0] + rng.uniform(-shift, shift, s.eps[0].shape), axis=1)
+    wfc1 = np.empty_like(s.wfc[0])
+    for ik in range(nk):
+        wfc1[ik] = _random_unitary(rng, s.wfc.shape[3]).T
+    eps = np.stack([s.eps[0], eps1])
+    occ = np.zeros((2, nk, nb))
+    occ[:, :, :nocc] = 1.0
+    homo = eps[:, :, nocc - 1].max()
+    lumo = eps[:, :, nocc].min()
+    if not homo < lumo:
+        raise ValueError("spin_polarised: the shift closes the gap")
+    return dataclasses.replace(s, name=s.name + "_spin", eps=eps, occ=occ, wfc=np.stack([s.wfc[0], wfc1]),
+                               efermi=float(0.5 * (homo + lumo)), nspin=2)
+
+
 WORKLOADS = {
     "cell2": cell2,
     "cell16": cell16,

@@ -77,16 +77,16 @@ def fold_cs(sys) -> Dict[Tuple[int, int, Tuple[int, int, int]], np.ndarray]:
     return out
 
 
-def occ_weights(sys, tau, nbands_g=-1):
-    """Band scale factors s_{n,k}(tau) for one spin (chi0.cpp:333-355)."""
+def occ_weights(sys, tau, nbands_g=-1, ispin=0):
+    """Band scale factors s_{n,k}(tau) of spin ispin (chi0.cpp:333-355)."""
     nk = sys.nk
-    wg = sys.occ[0] / nk
+    wg = sys.occ[ispin] / nk
     if tau > 0:
         w = 1.0 / nk - wg * sys.nspin / 2.0
         w = np.where(w < 0, 0.0, w)
     else:
         w = wg * 0.5 * sys.nspin
-    sc = -tau * (sys.eps[0] - sys.efermi)
+    sc = -tau * (sys.eps[ispin] - sys.efermi)
     sc = np.where(sc > 0, 0.0, sc)
     s = np.exp(sc) * w
     if nbands_g >= 0:
@@ -94,10 +94,10 @@ def occ_weights(sys, tau, nbands_g=-1):
     return s
 
 
-def gf_R(sys, tau, Rs, nbands_g=-1):
-    """Dense G(R, tau) (nbasis x nbasis, real) for each R in Rs."""
-    s = occ_weights(sys, tau, nbands_g)
-    U = sys.wfc[0]
+def gf_R(sys, tau, Rs, nbands_g=-1, ispin=0):
+    """Dense G(R, tau) (nbasis x nbasis, real) of spin ispin for each R in Rs."""
+    s = occ_weights(sys, tau, nbands_g, ispin)
+    U = sys.wfc[ispin]
     M = np.stack([(U[k].T * s[k][None, :]) @ U[k].conj() for k in range(sys.nk)])
     out = {}
     for R in Rs:
@@ -187,16 +187,22 @@ def upper_pairs(natom):
 
 
 def chi0s_all_tau(sys, grid, nbands_g=-1, method='factorized'):
-    """chi0s[it][(I, J, R)] for all minimax tau points, I <= J, R on the BvK grid."""
+    """chi0s[it][(I, J, R)] for all minimax tau points, I <= J, R on the BvK grid, summed over
+    the spin channels (Chi0::build loops over spins into one chi0s, chi0.cpp:57-135; the 2 / N_s
+    factor is applied in the transform)."""
     cs = fold_cs(sys)
     Rs = r_grid(sys.period)
     pairs = upper_pairs(sys.natom)
     f = chi0s_factorized if method == 'factorized' else chi0s_bruteforce
     out = []
     for tau in grid['tau']:
-        Gp = gf_R(sys, tau, Rs, nbands_g)
-        Gn = gf_R(sys, -tau, Rs, nbands_g)
-        out.append(f(sys, cs, Gp, Gn, pairs, Rs))
+        acc = None
+        for isp in range(sys.nspin):
+            Gp = gf_R(sys, tau, Rs, nbands_g, isp)
+            Gn = gf_R(sys, -tau, Rs, nbands_g, isp)
+            c = f(sys, cs, Gp, Gn, pairs, Rs)
+            acc = c if acc is None else {k: acc[k] + c[k] for k in acc}
+        out.append(acc)
     return out
 
 

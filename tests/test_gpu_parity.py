@@ -62,6 +62,9 @@ CASES = {
     "tiny_gamma": lambda: synth.tiny(natom=3, period=(1, 1, 1), nao=(3, 4), nabf=(5, 6), nocc=(1, 2), a=9.0,
                                      seed=5, nfreq=6),
     "cell2_k8": lambda: synth.cell2(),
+    "tiny_k4_3atoms_spin": lambda: synth.spin_polarised(
+        synth.tiny(natom=3, period=(1, 2, 2), nao=(2, 3), nabf=(3, 4), nocc=(1, 1), a=6.5, seed=11, nfreq=8)),
+    "cell2_k8_spin": lambda: synth.spin_polarised(synth.cell2(), seed=3),
 }
 
 

@@ -143,3 +143,27 @@ def test_lu_logdet_imaginary_part_is_pivot_sign_artefact():
             assert abs(ld.real - logabs) < 1e-10
             m = ld.imag / np.pi
             assert abs(m - round(m)) < 1e-8
+
+
+def test_spin_polarised_occupation_and_sum():
+    # two spin channels: Tr G_s(0, 0-) = -n_occ per spin (occupations 1), and the spin-summed
+    # chi0s equals the sum of two single-channel builds
+    s = synth.spin_polarised(_sys(natom=2, period=(2, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=7.0))
+    Rs = rpa.r_grid(s.period)
+    nocc = int(s.occ[0, 0].sum())
+    for isp in range(2):
+        Gn = rpa.gf_R(s, -1e-12, Rs, ispin=isp)
+        Gp = rpa.gf_R(s, 1e-12, Rs, ispin=isp)
+        assert abs(np.trace(Gn[(0, 0, 0)]) + nocc) < 1e-10
+        d = Gp[(0, 0, 0)] - Gn[(0, 0, 0)]
+        assert np.abs(d - np.eye(s.nbasis)).max() < 1e-10
+    emin, emax = tfgrids.e_min_max(s.eps, s.occ, s.nspin, s.nk)
+    g = tfgrids.minimax(6, emin, emax)
+    both = rpa.chi0s_all_tau(s, g)
+    cs = rpa.fold_cs(s)
+    pairs = rpa.upper_pairs(s.natom)
+    tau = g["tau"][2]
+    parts = [rpa.chi0s_factorized(s, cs, rpa.gf_R(s, tau, Rs, ispin=i), rpa.gf_R(s, -tau, Rs, ispin=i), pairs, Rs)
+             for i in range(2)]
+    for k in both[2]:
+        assert np.abs(both[2][k] - parts[0][k] - parts[1][k]).max() < 1e-13 * max(1.0, np.abs(both[2][k]).max())
