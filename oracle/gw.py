@@ -291,8 +291,8 @@ def sigma_all(sys, grid, wrt, spin=0, method='factorized'):
     sp, sn = [], []
     for it, tau in enumerate(grid['tau']):
         Wt = {R: wrt[(it, R)] for R in Rs}
-        sp.append(f(sys, cs, gf_R(sys, tau, Rs), Wt))
-        sn.append(f(sys, cs, gf_R(sys, -tau, Rs), Wt))
+        sp.append(f(sys, cs, gf_R(sys, tau, Rs, ispin=spin), Wt))
+        sn.append(f(sys, cs, gf_R(sys, -tau, Rs, ispin=spin), Wt))
     return sigma_freq(sys, grid, sp, sn)
 
 
@@ -313,8 +313,8 @@ def nearest_image(frac, latvec, I, J, R, period):
     return Rb
 
 
-def sigma_ks_diag(sys, sig_w, iks, states, frac=None):
-    """Sigma_KS(k, i w)_nn: dict (ik, n) -> complex array over omega (ascending)."""
+def sigma_ks_diag(sys, sig_w, iks, states, frac=None, spin=0):
+    """Sigma_KS(k, i w)_nn of one spin: dict (ik, n) -> complex array over omega (ascending)."""
     ao = Basis(sys.nao)
     per = sys.period
     Rs = r_grid(per)
@@ -322,7 +322,7 @@ def sigma_ks_diag(sys, sig_w, iks, states, frac=None):
     out = {}
     for ik in iks:
         kf = sys.kfrac[ik]
-        U = sys.wfc[0, ik]
+        U = sys.wfc[spin, ik]
         vals = np.zeros((len(states), nw), dtype=np.complex128)
         for iw in range(nw):
             Sk = np.zeros((ao.total, ao.total), dtype=np.complex128)

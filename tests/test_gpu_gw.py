@@ -98,6 +98,50 @@ def test_g0w0_parity(case):
     assert num / den < 1e-10
 
 
+def test_g0w0_parity_spin_polarised():
+    # two spin channels: W from the spin-summed chi0, Sigma_c and its KS diagonal per spin
+    _say("g0w0 parity, spin-polarised tiny k-point system")
+    s = synth.spin_polarised(CASES["tiny_k2_mixed"]())
+    h, o, grid = _lib(s)
+    chi0s = rpa.chi0s_all_tau(s, grid)
+    wrt = gw.w_r_tau(s, grid, gw.wc_q_omega(s, grid, rpa.chi0_q_omega(s, grid, chi0s)))
+    abf, ao = rpa.Basis(s.nabf), rpa.Basis(s.nao)
+    Rs = rpa.r_grid(s.period)
+    num = den = 0.0
+    for it in range(s.nfreq):
+        for R in Rs:
+            ref = wrt[(it, R)]
+            for I in range(s.natom):
+                for J in range(s.natom):
+                    got = h.get_wc_block(it, I, J, R, int(s.nabf[I]), int(s.nabf[J]))
+                    num = max(num, np.abs(got - ref[abf.sl(I), abf.sl(J)]).max())
+            den = max(den, np.abs(ref).max())
+    _say(f"  W(R,tau) rel err {num / den:.3e}")
+    assert num / den < 1e-10
+    frac = s.positions @ np.linalg.inv(s.latvec)
+    for spin in range(2):
+        sig = gw.sigma_all(s, grid, wrt, spin=spin)
+        num = den = 0.0
+        for iw in range(s.nfreq):
+            for R in Rs:
+                ref = sig[(iw, R)]
+                for I in range(s.natom):
+                    for J in range(s.natom):
+                        got = h.get_sigma_block(spin, iw, I, J, R, int(s.nao[I]), int(s.nao[J]))
+                        num = max(num, np.abs(got - ref[ao.sl(I), ao.sl(J)]).max())
+                den = max(den, np.abs(ref).max())
+        _say(f"  spin {spin}: Sigma(R,iw) rel err {num / den:.3e}")
+        assert num / den < 1e-10
+        ks = gw.sigma_ks_diag(s, sig, [0, 1], [0, 1, 2], frac=frac, spin=spin)
+        num = den = 0.0
+        for (ik, st), ref in ks.items():
+            got = h.get_sigc_ks(spin, ik, st, s.nfreq)
+            num = max(num, np.abs(got - ref).max())
+            den = max(den, np.abs(ref).max())
+        _say(f"  spin {spin}: Sigma_KS rel err {num / den:.3e}")
+        assert num / den < 1e-10
+
+
 @pytest.mark.parametrize("case", ["tiny_k2_mixed", "cell2_k8"])
 def test_g0w0_quasiparticle(case):
     """Pade continuation and the QP fixed point on the library's Sigma_KS(i omega).
