@@ -369,20 +369,24 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long s
     }
 }
 
-// Qs[r][c] = Q[r][c] * sgn[c] for matrix blockIdx.y of the batch
-__global__ void scale_cols_kernel(const double *__restrict__ panel, const double *__restrict__ work, int n, int m,
-                                  int nb) {
+// Qs[r][c] = Q[r][c] * sgn[c] for matrix blockIdx.y of the batch; Q and Qs are m x nb regions of
+// pitch CHOL_PW inside that matrix's pair of wide panels
+constexpr int CHOL_PW = 4 * CHOL_NB;  // outer panel width: the trailing update runs with K = 256
+
+__global__ void scale_cols_kernel(double *__restrict__ Q, double *__restrict__ Qs, long long pstride,
+                                  const double *__restrict__ work, int m, int nb) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)m * nb) return;
-    const double *Q = panel + (long long)blockIdx.y * 2 * n * CHOL_NB;
-    double *Qs = const_cast<double *>(Q) + (long long)n * CHOL_NB;
+    const long long r = idx / nb, c = idx % nb;
     const double *sgn = work + (long long)blockIdx.y * (CHOL_NB * CHOL_NB + CHOL_NB) + CHOL_NB * CHOL_NB;
-    Qs[idx] = Q[idx] * sgn[idx % nb];
+    const long long o = (long long)blockIdx.y * pstride + r * CHOL_PW + c;
+    Qs[o] = Q[o] * sgn[c];
 }
 
 struct CholPlan {
     std::vector<int> j0s, nbs;
-    std::vector<GemmBatch> trsm, syrk;
+    std::vector<GemmBatch> trsm, inpanel;  // per 64-column step
+    std::vector<GemmBatch> trail;          // per step: the K = 256 trailing update after an outer panel
 };
 
 static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan> &chol_plans() {
@@ -390,41 +394,65 @@ static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan
     return p;
 }
 
+// Right-looking blocked L S L^T factorisation, two levels: 64-column steps (diagonal block in
+// LDS, Q = A21 L11^-T by GEMM, signs applied) update only the rest of their 256-column outer
+// panel; after each outer panel one lower-triangular GEMM with K = 256 updates the trailing
+// matrix.  Q and Q S of the outer panel's rows are kept in two wide buffers (pitch 256, indexed
+// by global row) per matrix.
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
                              long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols) {
     const int nf = (ncols > 0 && ncols < n) ? ncols : n;
     const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
-    const long long pstride = 2LL * n * CHOL_NB;
+    const long long pstride = 2LL * n * CHOL_PW;  // Q then Q S, each n x 256
     w.linv.reserve(sizeof(double) * wstride * nbatch);
     w.panel.reserve(sizeof(double) * pstride * nbatch);
+    const long long qs_off = (long long)n * CHOL_PW;
     auto key = std::make_tuple(n, lda, nbatch, strideA, s, nf);
     auto &plans = chol_plans();
     auto it = plans.find(key);
     if (it == plans.end()) {
         CholPlan cp;
-        for (int j0 = 0; j0 < nf; j0 += CHOL_NB) {
-            const int nb = std::min(CHOL_NB, nf - j0);
-            const int m = n - j0 - nb;
-            cp.j0s.push_back(j0);
-            cp.nbs.push_back(nb);
-            GemmBatch tr, sy;
-            if (m > 0) {
-                tr.trans_b = true;
-                sy.trans_b = true;
-                sy.lower_only = true;
-                for (int b = 0; b < nbatch; ++b) {
-                    // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)); L21 = Q S
-                    tr.add(make_prob(m, nb, nb, strideA * b + (long long)(j0 + nb) * lda + j0, lda, wstride * b,
-                                     CHOL_NB, pstride * b, nb));
-                    // A22 -= (Q S) Q^T (lower tiles)
-                    sy.add(make_prob(m, m, nb, pstride * b + (long long)n * CHOL_NB, nb, pstride * b, nb,
-                                     strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
+        for (int p0 = 0; p0 < nf; p0 += CHOL_PW) {
+            const int pend = std::min(p0 + CHOL_PW, nf);
+            for (int j0 = p0; j0 < pend; j0 += CHOL_NB) {
+                const int nb = std::min(CHOL_NB, pend - j0);
+                const int m = n - j0 - nb;
+                const int wcols = pend - j0 - nb;  // panel columns right of this step
+                cp.j0s.push_back(j0);
+                cp.nbs.push_back(nb);
+                GemmBatch tr, ip, tl;
+                if (m > 0) {
+                    tr.trans_b = true;
+                    ip.trans_b = true;
+                    for (int b = 0; b < nbatch; ++b) {
+                        const long long wq = pstride * b + (long long)(j0 + nb) * CHOL_PW + (j0 - p0);
+                        // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)) into the wide buffer
+                        tr.add(make_prob(m, nb, nb, strideA * b + (long long)(j0 + nb) * lda + j0, lda, wstride * b,
+                                         CHOL_NB, wq, CHOL_PW));
+                        // A[j0+nb:, j0+nb:pend] -= (Q S) Q[0:wcols]^T
+                        if (wcols > 0)
+                            ip.add(make_prob(m, wcols, nb, wq + qs_off, CHOL_PW, wq, CHOL_PW,
+                                             strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
+                    }
+                    tr.finalize(T64x64, s);
+                    if (wcols > 0) ip.finalize(T64x64, s);
                 }
-                tr.finalize(T64x64, s);
-                sy.finalize(T64x64, s);
+                const int mt = n - pend;
+                if (j0 + nb == pend && mt > 0) {
+                    // trailing update of the outer panel: A22 -= (Q S) Q^T over its pend - p0 columns
+                    tl.trans_b = true;
+                    tl.lower_only = true;
+                    for (int b = 0; b < nbatch; ++b) {
+                        const long long wq = pstride * b + (long long)pend * CHOL_PW;
+                        tl.add(make_prob(mt, mt, pend - p0, wq + qs_off, CHOL_PW, wq, CHOL_PW,
+                                         strideA * b + (long long)pend * lda + pend, lda));
+                    }
+                    tl.finalize(T128x128w8, s);
+                }
+                cp.trsm.push_back(std::move(tr));
+                cp.inpanel.push_back(std::move(ip));
+                cp.trail.push_back(std::move(tl));
             }
-            cp.trsm.push_back(std::move(tr));
-            cp.syrk.push_back(std::move(sy));
         }
         it = plans.emplace(key, std::move(cp)).first;
     }
@@ -433,20 +461,24 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
     double *panel = w.panel.as<double>();
     for (size_t b = 0; b < cp.j0s.size(); ++b) {
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
+        const int p0 = (j0 / CHOL_PW) * CHOL_PW;
         hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(256), 0, s, A, strideA, lda, j0, nb, work, logdet,
                            ld_stride, flag);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
-        if (m <= 0) break;
-        gemm_run(Prec::F64, cp.trsm[b], A, work, panel, 1.0, 0.0, s);
-        hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256), nbatch), dim3(256), 0, s, panel,
-                           work, n, m, nb);
-        HIP_CHECK(hipGetLastError());
-        if (keep_L)
-            for (int bb = 0; bb < nbatch; ++bb)
-                launch_copy_block(panel + pstride * bb + (long long)n * CHOL_NB, nb,
-                                  A + strideA * bb + (long long)(j0 + nb) * lda + j0, lda, m, nb, s);
-        gemm_run(Prec::F64, cp.syrk[b], panel, panel, A, -1.0, 1.0, s);
+        if (m > 0) {
+            double *wq = panel + (long long)(j0 + nb) * CHOL_PW + (j0 - p0);
+            gemm_run(Prec::F64, cp.trsm[b], A, work, panel, 1.0, 0.0, s);
+            hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256), nbatch), dim3(256), 0, s, wq,
+                               wq + qs_off, pstride, work, m, nb);
+            HIP_CHECK(hipGetLastError());
+            if (keep_L)
+                for (int bb = 0; bb < nbatch; ++bb)
+                    launch_copy_block(wq + pstride * bb + qs_off, CHOL_PW, A + strideA * bb + (long long)(j0 + nb) * lda + j0,
+                                      lda, m, nb, s);
+            gemm_run(Prec::F64, cp.inpanel[b], panel, panel, A, -1.0, 1.0, s);
+        }
+        gemm_run(Prec::F64, cp.trail[b], panel, panel, A, -1.0, 1.0, s);
     }
     if (keep_L)
         for (int bb = 0; bb < nbatch; ++bb) launch_zero_upper<double>(A + strideA * bb, n, lda, s);
