@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long s
 
 // Qs[r][c] = Q[r][c] * sgn[c] for matrix blockIdx.y of the batch; Q and Qs are m x nb regions of
 // pitch CHOL_PW inside that matrix's pair of wide panels
-constexpr int CHOL_PW = 4 * CHOL_NB;  // outer panel width: the trailing update runs with K = 256
+constexpr int CHOL_PW = 8 * CHOL_NB;  // outer panel width: the trailing update runs with K = 512
 
 __global__ void scale_cols_kernel(double *__restrict__ Q, double *__restrict__ Qs, long long pstride,
                                   const double *__restrict__ work, int m, int nb) {
@@ -386,7 +386,7 @@ __global__ void scale_cols_kernel(double *__restrict__ Q, double *__restrict__ Q
 struct CholPlan {
     std::vector<int> j0s, nbs;
     std::vector<GemmBatch> trsm, inpanel;  // per 64-column step
-    std::vector<GemmBatch> trail;          // per step: the K = 256 trailing update after an outer panel
+    std::vector<GemmBatch> trail;          // per step: the K = CHOL_PW trailing update after an outer panel
 };
 
 static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan> &chol_plans() {
@@ -396,14 +396,14 @@ static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan
 
 // Right-looking blocked L S L^T factorisation, two levels: 64-column steps (diagonal block in
 // LDS, Q = A21 L11^-T by GEMM, signs applied) update only the rest of their 256-column outer
-// panel; after each outer panel one lower-triangular GEMM with K = 256 updates the trailing
-// matrix.  Q and Q S of the outer panel's rows are kept in two wide buffers (pitch 256, indexed
+// panel; after each outer panel one lower-triangular GEMM with K = CHOL_PW updates the trailing
+// matrix.  Q and Q S of the outer panel's rows are kept in two wide buffers (pitch CHOL_PW, indexed
 // by global row) per matrix.
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
                              long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols) {
     const int nf = (ncols > 0 && ncols < n) ? ncols : n;
     const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
-    const long long pstride = 2LL * n * CHOL_PW;  // Q then Q S, each n x 256
+    const long long pstride = 2LL * n * CHOL_PW;  // Q then Q S, each n x CHOL_PW
     w.linv.reserve(sizeof(double) * wstride * nbatch);
     w.panel.reserve(sizeof(double) * pstride * nbatch);
     const long long qs_off = (long long)n * CHOL_PW;
