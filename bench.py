@@ -312,6 +312,7 @@ def main():
         "t_ec_s": round(st["t_ec"], 4),
         "roofline": roof,
         "regions_ms": {k: round(v[0] / max(1, args.steps), 3) for k, v in prof.items()},
+        "regions_calls": {k: int(v[1]) // max(1, args.steps) for k, v in prof.items()},
         # per region: executed TFLOP/s and algorithmic GB/s (0 where a region counts no bytes)
         "regions_rate": {k: [round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0,
                              round(v[3] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else 0.0]

@@ -335,11 +335,12 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.d_jobs = upload(c.jobs, stream);
         c.d_colsrc = upload(c.colsrc, stream);
         flops_chi0_exec += c.wb.flops + c.xb.flops + c.ta.flops + c.fin_flops;
-        // algorithmic HBM bytes of one contraction launch: Cfin_I and T read once, chi0s read and written
-        c.fin_bytes = 0;
-        for (const auto &f : c.fin)
-            for (const auto &p : f.probs)
-                c.fin_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + 2.0 * p.M * p.N);
+        // algorithmic HBM bytes of each contraction launch: Cfin_I and T read once, chi0s read and written
+        for (int k = 0; k < 4; ++k) {
+            c.fin_bytes[k] = 0;
+            for (const auto &p : c.fin[k].probs)
+                c.fin_bytes[k] += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + 2.0 * p.M * p.N);
+        }
         c.ta_bytes = 0;
         for (const auto &p : c.ta.probs)
             c.ta_bytes += (double)ts * ((double)p.M * p.K + (double)p.K * p.N + (double)p.M * p.N);
@@ -545,9 +546,13 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 prof.begin("chi0_TA", stream);
                 gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                 prof.end("chi0_TA", stream, c.ta.flops, c.ta_bytes);
-                prof.begin("chi0_contract", stream);
-                for (auto &f : c.fin) gemm_run(prec, f, d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
-                prof.end("chi0_contract", stream, c.fin_flops, c.fin_bytes);
+                // one timing region per launch (per block-shape class)
+                for (int k = 0; k < 4; ++k) {
+                    if (c.fin[k].total_tiles == 0) continue;
+                    prof.begin("chi0_contract", stream);
+                    gemm_run(prec, c.fin[k], d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
+                    prof.end("chi0_contract", stream, c.fin[k].flops, c.fin_bytes[k]);
+                }
             }
         }
     }

@@ -42,7 +42,7 @@ struct RpaEngine {
         int j_begin, j_end;          // atoms J in [j_begin, j_end)
         int o_begin, o_end;          // outputs
         long long ghat_elems = 0, t_elems = 0;
-        double fin_bytes = 0, ta_bytes = 0;  // algorithmic HBM bytes per launch
+        double fin_bytes[4] = {0, 0, 0, 0}, ta_bytes = 0;  // algorithmic HBM bytes per launch
         std::vector<long long> ghat_off;  // per (J - j_begin, S, sigma)
         std::vector<GatherJob> jobs;
         std::vector<long long> colsrc;
