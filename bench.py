@@ -202,7 +202,9 @@ def g0w0_bench(args):
         h.build_g0w0_sigma(o)
         t = time.perf_counter() - t0
         prof = L.profile()
-        res["runs"][pn] = {"wall_s": round(t, 3), "regions_ms": {k: round(v[0], 2) for k, v in prof.items()}}
+        res["runs"][pn] = {"wall_s": round(t, 3), "regions_ms": {k: round(v[0], 2) for k, v in prof.items()},
+                           "regions_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0
+                                              for k, v in prof.items()}}
         _log(f"{pn}: G0W0 build {t:.2f} s")
     res["value"] = res["runs"][precs[0]]["wall_s"]
     print(json.dumps(res), flush=True)

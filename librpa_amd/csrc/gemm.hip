@@ -399,8 +399,9 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
     }
     if (b.tag == 3 || b.tag == 4) {
         // same code as the plain kernel, instantiated separately so that traces can tell it apart
-        if (b.trans_a || b.trans_b) LRA_THROW("tagged GEMM: plain A and B");
-        if (b.tag == 3) launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
+        if (b.trans_b || (b.trans_a && b.tag == 4)) LRA_THROW("tagged GEMM: plain B (and plain A for tag 4)");
+        if (b.tag == 3 && b.trans_a) launch_cfg<MODE, true, false, 3>(b, A, B, C, alpha, beta, s);
+        else if (b.tag == 3) launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
         else launch_cfg<MODE, false, false, 4>(b, A, B, C, alpha, beta, s);
         return;
     }

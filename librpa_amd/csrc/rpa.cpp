@@ -122,8 +122,8 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                     for (int k = 0; k < nkk; ++k)
                         for (int mu = 0; mu < nm; ++mu) {
                             const double v = blk[((size_t)i * nkk + k) * nm + mu];
-                            // Cfin_I[mu][koff + k * nb(I) + i]
-                            cfin[cfin_off[I] + (size_t)mu * ksum[I] + koff[I][n] + (size_t)k * ni + i] = v;
+                            // Cfin_I stored transposed: [koff + k * nb(I) + i][mu]
+                            cfin[cfin_off[I] + (size_t)(koff[I][n] + (size_t)k * ni + i) * nm + mu] = v;
                             // CWb_I[(loff + k)][i * na(I) + mu]   (I plays the role of J, K of L)
                             cwb[cwb_off[I] + (size_t)(loff[I][n] + k) * ni * nm + (size_t)i * nm + mu] = v;
                         }
@@ -315,7 +315,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
             const auto &O = outs[o];
             if (ksum[O.I] == 0) continue;
             const int cls = (na[O.I] > 64 ? 2 : 0) + (na[O.J] > 64 ? 1 : 0);
-            c.fin[cls].add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], (int)ksum[O.I], O.toff,
+            c.fin[cls].add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], na[O.I], O.toff,
                                      na[O.J], O.eoff, na[O.J]));
             c.fin_owner[cls].push_back(o);
         }
@@ -329,6 +329,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.fin_flops = 0;
         for (auto &f : c.fin) {
             f.tag = 3;
+            f.trans_a = true;  // Cfin_I is stored k-major, so A tiles load along mu
             f.finalize(T64x64, stream);
             c.fin_flops += f.flops;
         }
