@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -171,7 +172,12 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     // ---- chunking over J by memory budget (gathered G panels and T) ----
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const double budget = std::min(48e9, 0.35 * (double)free_b - (double)w_off.back() * ts);
+    double budget = std::min(48e9, 0.35 * (double)free_b - (double)w_off.back() * ts);
+    // LIBRPA_AMD_CHUNK_BYTES=<bytes> caps the chunk size (smaller chunks, same results)
+    if (const char *cb = std::getenv("LIBRPA_AMD_CHUNK_BYTES")) {
+        const double v = std::atof(cb);
+        if (v > 0) budget = std::min(budget, v);
+    }
     chunks.clear();
     {
         int j = 0, o = 0;
@@ -500,6 +506,18 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     d_Ghat.reserve(ghat_max * ts);
     d_W.reserve((size_t)w_off.back() * ts);
     d_T.reserve(t_max * ts);
+    if (const char *ov = std::getenv("LIBRPA_AMD_OVERLAP")) overlap_ta = std::atoi(ov) != 0;
+    if (overlap_ta && chunks.size() > 1) {
+        if (!s2) {
+            HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            for (hipEvent_t *e : {&ev_ready, &ev_s2, &ev_ta[0], &ev_ta[1], &ev_c[0], &ev_c[1]})
+                HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
+        // a second T buffer when the device has room for it (else the chunks run in sequence)
+        size_t free_b = 0, total_b = 0;
+        HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        if (d_T2.bytes() < t_max * ts && free_b > t_max * ts + (size_t)4e9) d_T2.reserve(t_max * ts);
+    }
     HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
     if (screen_mode != 0 && nspin > 1) LRA_THROW("chi0 block screening needs nspin = 1");
     if (screen_mode == 2) {
@@ -532,6 +550,37 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 prof.begin("chi0_W", stream);
                 gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
                 prof.end("chi0_W", stream, c.wb.flops);
+            }
+            // T_A of chunk c + 1 (second stream, second T buffer) overlaps the contraction of chunk c
+            const bool overlap = overlap_ta && !tune_now && chunks.size() > 1 && d_T2.bytes() >= t_max * ts;
+            if (overlap) {
+                HIP_CHECK(hipEventRecord(ev_ready, stream));  // G and W of this tau written
+                HIP_CHECK(hipStreamWaitEvent(s2, ev_ready, 0));
+                for (size_t ci = 0; ci < chunks.size(); ++ci) {
+                    auto &c = chunks[ci];
+                    void *Tb = (ci % 2) ? d_T2.get() : d_T.get();
+                    // the contraction two chunks back has finished reading this buffer
+                    if (ci >= 2) HIP_CHECK(hipStreamWaitEvent(s2, ev_c[ci % 2], 0));
+                    prof.begin("chi0_TA", s2);
+                    gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), Tb, 1.0, 0.0, s2);
+                    prof.end("chi0_TA", s2, c.ta.flops, c.ta_bytes);
+                    HIP_CHECK(hipEventRecord(ev_ta[ci % 2], s2));
+                    prof.begin("chi0_X", stream);
+                    gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                    prof.end("chi0_X", stream, c.xb.flops);
+                    HIP_CHECK(hipStreamWaitEvent(stream, ev_ta[ci % 2], 0));
+                    for (int k = 0; k < 4; ++k) {
+                        if (c.fin[k].total_tiles == 0) continue;
+                        prof.begin("chi0_contract", stream);
+                        gemm_run(prec, c.fin[k], d_Cfin.get(), Tb, chi_base, 1.0, 1.0, stream);
+                        prof.end("chi0_contract", stream, c.fin[k].flops, c.fin_bytes[k]);
+                    }
+                    HIP_CHECK(hipEventRecord(ev_c[ci % 2], stream));
+                }
+                // the next G and W builds overwrite what T_A reads
+                HIP_CHECK(hipEventRecord(ev_s2, s2));
+                HIP_CHECK(hipStreamWaitEvent(stream, ev_s2, 0));
+                continue;
             }
             for (auto &c : chunks) {
                 if (tune_now && !c.ta.tuned) {
@@ -775,6 +824,16 @@ void RpaEngine::release_work() {
     d_Ghat.release();
     d_W.release();
     d_T.release();
+    d_T2.release();
+}
+
+RpaEngine::~RpaEngine() {
+    if (s2) {
+        (void)hipStreamSynchronize(s2);
+        for (hipEvent_t e : {ev_ready, ev_s2, ev_ta[0], ev_ta[1], ev_c[0], ev_c[1]})
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(s2);
+    }
 }
 
 void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {

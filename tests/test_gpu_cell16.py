@@ -97,6 +97,23 @@ def test_cell16_fp64_matches_oracle(system, golden, fp64):
 
 
 @pytest.mark.gpu
+def test_cell16_small_chunks_match_oracle(system, golden, fp64, monkeypatch):
+    # chunks of a few atoms each: the T_A / contraction overlap across chunks and the chunk
+    # bookkeeping must reproduce the single-chunk result
+    d, _ = golden
+    monkeypatch.setenv("LIBRPA_AMD_CHUNK_BYTES", "2e8")
+    monkeypatch.setenv("LIBRPA_AMD_OVERLAP", "1")
+    _say("cell16 fp64 run with small chunks")
+    h, o, ec, contrib = _run(system, L.FP64)
+    _say(f"  Ec {ec!r} oracle {d['ec_ha']!r} single-chunk {fp64['ec']!r}")
+    assert abs(ec - d["ec_ha"]) < 1e-8
+    for it in (0, 12):
+        ref = fp64["blocks"][it]
+        got = _blocks(h, system, it)
+        assert max(np.abs(got[k] - ref[k]).max() for k in ref) <= 1e-12 * max(np.abs(b).max() for b in ref.values())
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("prec,tol_chi", [(L.FP32, 1e-4), (L.BF16X3, 1e-3)])
 def test_cell16_reduced_precision_report(system, golden, fp64, prec, tol_chi):
     d, _ = golden
