@@ -243,14 +243,29 @@ def ln_det_ref(A):
     return np.sum(np.where(d.real > 0, np.log(d.astype(complex)), np.log(-d.astype(complex))))
 
 
-def rpa_ec(sys, grid, chi0q, vq=None):
-    """Ec and per-k complex contributions."""
+def ibz_weights(nk, ibz_map=None):
+    """IBZ k list in order of first appearance and weights count / N_k
+    (PeriodicBoundaryData::set_ibz_mapping, src/core/pbc.cpp:110-153)."""
+    m = list(range(nk)) if ibz_map is None else [int(x) for x in ibz_map]
+    ks, w = [], []
+    for k in range(nk):
+        if m[k] in ks:
+            w[ks.index(m[k])] += 1.0 / nk
+        else:
+            ks.append(m[k])
+            w.append(1.0 / nk)
+    return ks, w
+
+
+def rpa_ec(sys, grid, chi0q, vq=None, ibz_map=None):
+    """Ec and the contributions of the IBZ k points (all k without a mapping), each weighted by
+    its star's share of the k grid (epsilon.cpp:248, map_ibzk_weight)."""
     abf = Basis(sys.nabf)
     vq = sys.vq if vq is None else vq
     gamma_only = sys.nk == 1
-    nq = sys.nk
-    contrib = np.zeros(nq, dtype=np.complex128)
-    for iq in range(nq):
+    qs, wts = ibz_weights(sys.nk, ibz_map)
+    contrib = np.zeros(len(qs), dtype=np.complex128)
+    for a, iq in enumerate(qs):
         V = vq[iq]
         for iw in range(len(grid['omega'])):
             chi = full_matrix(chi0q, abf, sys.natom, lambda I, J: chi0q[(iw, iq, I, J)])
@@ -264,7 +279,7 @@ def rpa_ec(sys, grid, chi0q, vq=None):
                 val = complex(tr + ld)
             else:
                 val = tr + ld
-            contrib[iq] += val * grid['omega_w'][iw] * (1.0 / nq) / (2 * np.pi)
+            contrib[a] += val * grid['omega_w'][iw] * wts[a] / (2 * np.pi)
     return float(contrib.sum().real), contrib
 
 

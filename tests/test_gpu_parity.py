@@ -120,3 +120,24 @@ def test_reduced_precision_cell2():
         _say(f"  prec={prec} chi0s rel err={err:.2e} Ec={ec!r} fp64={ec64!r} dEc/atom={(ec - ec64) / s.natom:.3e}")
         assert err < tol_chi
         assert np.isfinite(ec)
+
+
+@pytest.mark.parametrize("case,ibz", [("tiny_k3_odd", [0, 1, 1]), ("cell2_k8", [0, 0, 2, 2, 4, 4, 4, 7])])
+def test_ibz_mapping(case, ibz):
+    # Ec over the IBZ list of an irreducible-k mapping, each k weighted by its star
+    _say(f"ibz mapping case={case} map={ibz}")
+    s = CASES[case]()
+    h = L.Handler()
+    L.load_system(h, s)
+    h.set_ibz_mapping(ibz)
+    nibz = len(set(ibz))
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    ec, contrib = h.get_rpa_correlation_energy(o, nibz)
+    grid = h.get_tfgrids(o)
+    chi0s = rpa.chi0s_all_tau(s, grid)
+    q = rpa.chi0_q_omega(s, grid, chi0s)
+    ec_ref, c_ref = rpa.rpa_ec(s, grid, q, ibz_map=ibz)
+    _say(f"  Ec lib={ec!r} oracle={ec_ref!r}")
+    assert len(contrib) == nibz
+    assert abs(ec - ec_ref) < 1e-7
+    assert np.abs(contrib.real - c_ref.real).max() < 1e-7
