@@ -141,3 +141,27 @@ def test_ibz_mapping(case, ibz):
     assert len(contrib) == nibz
     assert abs(ec - ec_ref) < 1e-7
     assert np.abs(contrib.real - c_ref.real).max() < 1e-7
+
+
+@pytest.mark.parametrize("case,nb", [("tiny_k4_3atoms", 5), ("cell2_k8", 20)])
+def test_band_cut_n_bands_chi0(case, nb):
+    # n_bands_chi0 >= 0 restricts G(R, tau) to the lowest nb bands (chi0.cpp:378-383)
+    _say(f"n_bands_chi0 case={case} nb={nb}")
+    s = CASES[case]()
+    h = L.Handler()
+    L.load_system(h, s)
+    h.set_keep_chi0(True)
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI, n_bands_chi0=nb)
+    ec, contrib = h.get_rpa_correlation_energy(o, s.nk)
+    grid = h.get_tfgrids(o)
+    chi0s = rpa.chi0s_all_tau(s, grid, nbands_g=nb)
+    ec_ref, _ = rpa.rpa_ec(s, grid, rpa.chi0_q_omega(s, grid, chi0s))
+    num = den = 0.0
+    for it in range(s.nfreq):
+        for (I, J, R), ref in chi0s[it].items():
+            got = h.get_chi0s_block(it, I, J, R, int(s.nabf[I]), int(s.nabf[J]))
+            num = max(num, np.abs(got - ref).max())
+            den = max(den, np.abs(ref).max())
+    _say(f"  chi0s rel {num / den:.2e}  Ec lib={ec!r} oracle={ec_ref!r}")
+    assert num / den < 1e-12
+    assert abs(ec - ec_ref) < 1e-7
