@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -236,7 +237,8 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                                        (int)R0.lsum[J], R0.cwb_off[J], nb[J] * na[J], u_off[(size_t)J * nR + S],
                                        nb[J] * na[J]));
                 }
-            c.ub.finalize(big_tile(prec), st);
+            // default tiles: the autotuner's choices on the 64-atom, 8-k cell (FP64)
+            c.ub.finalize(prec == Prec::F64 ? T128x64w8 : big_tile(prec), st);
             flops_sigma += c.ub.flops;
             uchunks.push_back(std::move(c));
             j = jj;
@@ -283,7 +285,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                 const int L = R0.nbrs[J][n].K;
                 for (int l = 0; l < nb[L]; ++l) b2.coltab.push_back((long long)S * NAO + ao[L] + l);
             }
-            b1.finalize(T32x128, st);
+            b1.finalize(prec == Prec::F64 ? T16x256 : T32x128, st);
             b2.finalize(big_tile(prec), st);
             flops_sigma += b1.flops + b2.flops;
         }
@@ -367,7 +369,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                         q.M = m;
                         c.tdb.add(q);
                     }
-            c.tdb.finalize(T128x64, st);
+            c.tdb.finalize(prec == Prec::F64 ? T128x64w8 : T128x64, st);
             // label a0b0_a2b2: Sigma_I.(R)[i][(R, J, j)] = CD_I * T_I
             for (int I = c.i0; I < c.i1; ++I) {
                 GemmProb q = make_prob(nb[I], (int)LD, (int)(na[I] * lsum[I]), cd_off[I], (int)(na[I] * lsum[I]), tdo(I),
@@ -376,7 +378,7 @@ void GwEngine::plan(const Dataset &ds, std::shared_ptr<RpaEngine> r) {
                 q.cbs = (long long)NAO * NAO;
                 c.dcon.add(q);
             }
-            c.dcon.finalize(T32x128, st);
+            c.dcon.finalize(prec == Prec::F64 ? T32x64 : T32x128, st);
             // label a0b0_a1b2 from the on-site rows of T_I
             c.a3b.atomic = true;
             c.a3b.coltab = ext_cols;
@@ -682,6 +684,10 @@ void GwEngine::build_sigma(const Dataset &ds, const TFGrid &g) {
     (void)gpm_r;
     const size_t esz = ts;
     const long long nn_abf = (long long)NABF * NABF;
+    // LIBRPA_AMD_TUNE_SIGMA=1 times the Sigma batches' candidate tiles on the first pass (the
+    // trials cost about as much as they save in one G0W0 run, so the defaults are used otherwise)
+    const char *tsg = std::getenv("LIBRPA_AMD_TUNE_SIGMA");
+    const bool tune_sigma = R0.autotune && tsg && std::atoi(tsg) != 0;
     for (int it = 0; it < g.n; ++it) {
         const void *Wf;
         if (prec == Prec::F64) {
@@ -718,8 +724,44 @@ void GwEngine::build_sigma(const Dataset &ds, const TFGrid &g) {
                                                   c.d_colsrc[sg].as<long long>(), NAO, c.max_gcol, st);
                     prof.end("gw_gather", st, 0, 2.0 * c.ghat_elems * esz);
                     prof.begin("gw_U", st);
+                    if (tune_sigma && !sigma_tuned)
+                        c.ub.tune(prec, d_Ghat.get(), R0.d_CWb.get(), d_U.get(), 1.0, 0.0, st);
                     gemm_run(prec, c.ub, d_Ghat.get(), R0.d_CWb.get(), d_U.get(), 1.0, 0.0, st);
                     prof.end("gw_U", st, c.ub.flops);
+                }
+                if (tune_sigma && !sigma_tuned) {
+                    // time the candidate tiles once per batch shape; the trials leave garbage in
+                    // GC, Y, T^D and Sigma, which are reset before the real pass
+                    for (int J = 0; J < natom; ++J) {
+                        const char *gbase = R0.d_Gpm.as<char>() + (size_t)sg * R0.nb[J] * esz;
+                        for (int rr = 0; rr < nR; ++rr) {
+                            GemmBatch &b1 = a1[(size_t)J * nR + rr];
+                            GemmBatch &b2 = a2[(size_t)J * nR + rr];
+                            if (b1.empty()) continue;
+                            b1.tune(prec, gbase, d_CA.get(), d_GC.get(), 1.0, 0.0, st);
+                            gemm_run(prec, b1, gbase, d_CA.get(), d_GC.get(), 1.0, 0.0, st);
+                            b2.tune(prec, Wf, d_GC.get(), d_Y.get(), 1.0, 1.0, st);
+                        }
+                    }
+                    for (auto &c : ichunks) {
+                        c.tdb.tune(prec, Wf, d_U.get(), d_TD.get(), 1.0, 0.0, st);
+                        gemm_run(prec, c.tdb, Wf, d_U.get(), d_TD.get(), 1.0, 0.0, st);
+                        c.dcon.tune(prec, d_CD.get(), d_TD.get(), d_sig[sg].get(), 1.0, 1.0, st);
+                    }
+                    HIP_CHECK(hipMemsetAsync(d_Y.get(), 0, d_Y.bytes(), st));
+                    HIP_CHECK(hipMemsetAsync(d_sig[sg].get(), 0, d_sig[sg].bytes(), st));
+                    sigma_tuned = true;
+                    if (verbose) {
+                        auto nm = [](const GemmBatch &b) { return tile_name(b.cfg); };
+                        for (auto &c : uchunks) fprintf(stderr, "[librpa-mi355x] tile U %s\n", nm(c.ub));
+                        for (int J = 0; J < natom; ++J)
+                            if (!a1[(size_t)J * nR].empty())
+                                fprintf(stderr, "[librpa-mi355x] tile A1 J=%d %s A2 %s\n", J, nm(a1[(size_t)J * nR]),
+                                        nm(a2[(size_t)J * nR]));
+                        for (auto &c : ichunks)
+                            fprintf(stderr, "[librpa-mi355x] tile TD %s D %s\n", nm(c.tdb), nm(c.dcon));
+                        fflush(stderr);
+                    }
                 }
                 // partner-partner term: Y_I over all (J, R), then one contraction
                 prof.begin("gw_A12", st);

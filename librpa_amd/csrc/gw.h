@@ -77,6 +77,7 @@ struct GwEngine {
     double flops_sigma = 0, flops_w = 0, t_w = 0, t_sigma = 0;
     long long max_ghat = 0, max_tdc = 0;
     bool verbose = false;  // progress lines on stderr
+    bool sigma_tuned = false;  // Sigma batches' tiles timed (LIBRPA_AMD_TUNE_SIGMA=1) on the first pass
 
     void plan(const Dataset &ds, std::shared_ptr<RpaEngine> r);
     void build_w(const Dataset &ds, const TFGrid &g);
