@@ -286,6 +286,71 @@ def to_system(ds: dict, nfreq: int):
                         nspin=b["nspin"])
 
 
+def write_dataset(sys_, dir_path: str, nfreq: int = None, task: str = "rpa"):
+    """Writes a ``librpa_amd.synth.System`` as a LibRPA dataset directory in the ASCII formats the
+    readers above take (and LibRPA's driver reads, driver/read_data.cpp), plus a ``librpa.in``.
+
+    Seeded synthetic systems then run through the same file path as FHI-aims / ABACUS output:
+    ``python -m librpa_amd.driver <dir>/librpa.in``."""
+    os.makedirs(dir_path, exist_ok=True)
+    j = lambda f: os.path.join(dir_path, f)
+    ns, nk, nst, nb = sys_.nspin, sys_.nk, sys_.nbands, sys_.nbasis
+    ha2ev = 27.211386245988
+    with open(j("band_out"), "w") as f:
+        f.write(f"{nk}\n{ns}\n{nst}\n{nb}\n{sys_.efermi:.18e}\n")
+        for k in range(nk):
+            for s in range(ns):
+                f.write(f"{k + 1} {s + 1}\n")
+                for i in range(nst):
+                    e = sys_.eps[s, k, i]
+                    f.write(f"{i + 1} {sys_.occ[s, k, i]:.18e} {e:.18e} {e * ha2ev:.18e}\n")
+    lat = np.asarray(sys_.latvec, dtype=np.float64)
+    G = 2 * np.pi * np.linalg.inv(lat).T
+    with open(j("stru_out"), "w") as f:
+        for m in (lat, G):
+            for r in m:
+                f.write(" ".join(f"{x:.18e}" for x in r) + "\n")
+        f.write(f"{sys_.natom}\n")
+        for a in range(sys_.natom):
+            x = sys_.positions[a]
+            f.write(f"{x[0]:.18e} {x[1]:.18e} {x[2]:.18e} {int(sys_.types[a]) + 1}\n")
+    kcart = 2 * np.pi * np.linalg.solve(lat, np.asarray(sys_.kfrac, dtype=np.float64).T).T
+    with open(j("bz_sampling_out"), "w") as f:
+        f.write(" ".join(str(int(p)) for p in sys_.period) + "\n")
+        f.write(f"{nk} {nk}\n")
+        for k in range(nk):
+            kf, kc = sys_.kfrac[k], kcart[k]
+            f.write(f"{k + 1} {1.0 / nk:.18e} " + " ".join(f"{x:.18e}" for x in kf) + " " +
+                    " ".join(f"{x:.18e}" for x in kc) + f" {k + 1} {k + 1}\n")
+    types = sorted(set(int(t) for t in sys_.types))
+    with open(j("basis_out"), "w") as f:
+        f.write(f"{len(types)} {nb} {int(np.sum(sys_.nabf))} synthetic\n")
+        for t in types:
+            a = int(np.nonzero(np.asarray(sys_.types) == t)[0][0])
+            f.write(f"{t + 1} {int(sys_.nao[a])} {int(sys_.nabf[a])}\n")
+    with open(j("KS_eigenvector_0.txt"), "w") as f:
+        for k in range(nk):
+            f.write(f"{k + 1}\n")
+            v = sys_.wfc[:, k].transpose(2, 1, 0)  # (basis, state, spin)
+            for x in v.reshape(-1):
+                f.write(f"{x.real:.18e} {x.imag:.18e}\n")
+    with open(j("Cs_data_0.txt"), "w") as f:
+        f.write(f"{sys_.natom} {len(sys_.cs)}\n")
+        for (I, J, R, blk) in sys_.cs:
+            ni, nj, nmu = int(sys_.nao[I]), int(sys_.nao[J]), int(sys_.nabf[I])
+            f.write(f"{I + 1} {J + 1} {R[0]} {R[1]} {R[2]} {ni} {nj} {nmu}\n")
+            f.write("\n".join(f"{x:.18e}" for x in np.asarray(blk).reshape(-1)) + "\n")
+    n = int(np.sum(sys_.nabf))
+    with open(j("coulomb_mat_0.txt"), "w") as f:
+        f.write(f"{nk}\n")
+        for k in range(nk):
+            f.write(f"{n} 1 {n} 1 {n} {k + 1} {1.0 / nk:.18e}\n")
+            for x in np.asarray(sys_.vq[k]).reshape(-1):
+                f.write(f"{x.real:.18e} {x.imag:.18e}\n")
+    with open(j("librpa.in"), "w") as f:
+        f.write(f"task = {task}\ninput_dir = .\nnfreq = {nfreq or sys_.nfreq}\nparallel_routing = libri\n")
+
+
 ROUTING = {"auto": 0, "rtau": 1, "atompair": 3, "libri": 2}
 
 
