@@ -90,7 +90,8 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
                                                     const unsigned char *__restrict__ skip, int skip_zero) {
     using T = typename ModeTraits<MODE>::T;
     using V = typename ModeTraits<MODE>::V;
-    constexpr int BK = ModeTraits<MODE>::BK;
+    // the chi0 contraction (VAR 3, K ~ 25k) runs FP64 with a 32-deep K slice: half the barriers per flop
+    constexpr int BK = (VAR == 3 && MODE == 0) ? 32 : ModeTraits<MODE>::BK;
     constexpr int PAD = ModeTraits<MODE>::PAD;
     constexpr int LDK = BK + PAD;
     constexpr int NT = WM * WN * 64;
