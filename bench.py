@@ -74,7 +74,9 @@ def screening_report(L, h, o, s, ec64, thr=1e-8):
 
 
 def run_steps(L, h, o, nk, prec, steps, warmup):
-    h.set_precision(prec, prec)
+    # the mixed mode is a chi0 setting; with it the Ec transforms (chi0 L, L^T X) run in FP32, which
+    # the tau->omega transform does not amplify (the log-determinant stays FP64)
+    h.set_precision(prec, L.FP32 if prec == L.MIXED else prec)
     ec = None
     for _ in range(warmup):
         ec, _ = h.get_rpa_correlation_energy(o, nk)
@@ -108,6 +110,8 @@ def dominant_kernel(prof, prec_name, workload):
     cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_X", "chi0_contract",
                                                             "ec_gemm"))}
     name, (ms, calls, flops, nbytes) = max(cands.items(), key=lambda kv: kv[1][0])
+    if prec_name == "mixed":  # FP32: the tau points' regions with an _f32 suffix and the Ec GEMMs
+        prec_name = "fp32" if name.endswith("_f32") or name.startswith("ec_gemm") else "fp64"
     peak_f = PEAK_TFLOPS[prec_name]
     t_f = flops / (peak_f * 1e12)
     t_b = nbytes / (HBM_GBS * 1e9)
@@ -216,7 +220,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cubic128")
-    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3"])
+    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-screen", action="store_true", help="skip the block-norm screening report")
     ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
@@ -268,7 +272,7 @@ def main():
     s64 = float(np.median(t64))
     _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step  chi0 {st64['t_chi0']:.3f}s  Ec {st64['t_ec']:.3f}s")
 
-    cand = ["bf16x3", "fp32"] if args.prec == "auto" else [args.prec]
+    cand = ["bf16x3", "fp32", "mixed"] if args.prec == "auto" else [args.prec]
     chosen = None
     for pn in cand:
         if pn == "fp64":

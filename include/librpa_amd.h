@@ -13,6 +13,10 @@ extern "C" {
 #define LIBRPA_AMD_FP64 0
 #define LIBRPA_AMD_FP32 1
 #define LIBRPA_AMD_BF16X3 2
+/* chi0 only: FP64 at the tau points whose column of the cosine tau->omega
+ * matrix has an entry above 1e3 (the transform multiplies their chi0 errors
+ * by up to that much), FP32 at the others; chi0(R, tau) is stored in FP64. */
+#define LIBRPA_AMD_MIXED 3
 
 /* Contraction precision of the chi0 build (G, W, T and the final Cs.T
  * contraction) and of the Ec transforms (chi0 L and L^T X).  The Cholesky

@@ -499,10 +499,11 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             // the tau-invariant plan (packed Cs, problem lists, G-build operands) is reused
             // until a setter changes the inputs
             auto eng = d.rpa;
-            if (!eng || eng->prec != d.chi0_prec || eng->planned_ntau != d.grid.n) {
+            if (!eng || eng->prec != d.chi0_prec || eng->mixed != d.chi0_mixed || eng->planned_ntau != d.grid.n) {
                 d.rpa.reset();
                 d.gw.reset();
                 eng = std::make_shared<RpaEngine>();
+                eng->mixed = d.chi0_mixed;
                 eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
             }
             eng->ec_prec = d.ec_prec;
@@ -569,8 +570,9 @@ void librpa_get_g0w0_sigc_band_k(LibrpaHandler *h, const LibrpaOptions *, const 
 void librpa_amd_set_precision(LibrpaHandler *h, int chi0_prec, int ec_prec) {
     guarded_void([&]() {
         auto &d = ds_of(h);
-        if (chi0_prec < 0 || chi0_prec > 2 || ec_prec < 0 || ec_prec > 2) LRA_THROW("unknown precision");
-        d.chi0_prec = static_cast<Prec>(chi0_prec);
+        if (chi0_prec < 0 || chi0_prec > 3 || ec_prec < 0 || ec_prec > 2) LRA_THROW("unknown precision");
+        d.chi0_mixed = chi0_prec == 3;
+        d.chi0_prec = d.chi0_mixed ? Prec::F64 : static_cast<Prec>(chi0_prec);
         d.ec_prec = static_cast<Prec>(ec_prec);
     });
 }

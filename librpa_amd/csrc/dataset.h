@@ -66,6 +66,7 @@ struct Dataset {
     // precision / behaviour knobs of this build
     Prec chi0_prec = Prec::F64;
     Prec ec_prec = Prec::F64;
+    bool chi0_mixed = false;  // FP64 chi0 plan with FP32 contractions at the tau points of small weight
     bool keep_chi0 = false;
     int screen_mode = 0;
     double screen_thr = 1e-8;

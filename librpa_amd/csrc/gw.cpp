@@ -984,9 +984,10 @@ void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
         LRA_THROW("a positive sqrt_coulomb_threshold (eigenvalue filtering of v) is outside this build's path");
     lra_prepare_grid(d, o);
     auto eng = d.rpa;
-    if (!eng || eng->prec != d.chi0_prec || eng->planned_ntau != d.grid.n) {
+    if (!eng || eng->prec != d.chi0_prec || eng->mixed != d.chi0_mixed || eng->planned_ntau != d.grid.n) {
         d.rpa.reset();
         eng = std::make_shared<RpaEngine>();
+        eng->mixed = d.chi0_mixed;
         eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
         d.rpa = eng;
     }

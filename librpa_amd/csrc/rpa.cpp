@@ -133,6 +133,12 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         if (prec == Prec::F64) {
             d_Cfin = upload(cfin, stream);
             d_CWb = upload(cwb, stream);
+            if (mixed) {
+                std::vector<float> f1(cfin.begin(), cfin.end()), f2(cwb.begin(), cwb.end());
+                d_Cfin32 = upload(f1, stream);
+                d_CWb32 = upload(f2, stream);
+                HIP_CHECK(hipStreamSynchronize(stream));
+            }
         } else {
             std::vector<float> f1(cfin.begin(), cfin.end()), f2(cwb.begin(), cwb.end());
             d_Cfin = upload(f1, stream);
@@ -487,7 +493,7 @@ void RpaEngine::build_gf(const MeanField &mf, double tau, int s) {
             HIP_CHECK(hipMemsetAsync(d_Mst.get(), 0, d_Mst.bytes(), stream));
         }
         const double sgn = t < 0 ? -1.0 : 1.0;
-        if (prec == Prec::F64)
+        if (cur == Prec::F64)
             launch_k2r<double>(d_Mst.as<double>(), d_cs.as<double>(), d_sn.as<double>(),
                                d_gcol.as<int>() + (size_t)sg * NAO, d_Gpm.as<double>(), nk, nR, NAO, sgn, stream);
         else
@@ -506,6 +512,12 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     d_Ghat.reserve(ghat_max * ts);
     d_W.reserve((size_t)w_off.back() * ts);
     d_T.reserve(t_max * ts);
+    if (mixed) {
+        if (prec != Prec::F64) LRA_THROW("mixed chi0 precision needs the FP64 plan");
+        if (const char *e = std::getenv("LIBRPA_AMD_MIXED_AMAX")) mixed_amax = std::atof(e);
+        d_chi32.reserve(sizeof(float) * (size_t)E_tot);
+    }
+    cur = prec;
     if (const char *ov = std::getenv("LIBRPA_AMD_OVERLAP")) overlap_ta = std::atoi(ov) != 0;
     if (overlap_ta && chunks.size() > 1) {
         if (!s2) {
@@ -530,26 +542,42 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         c.xb.use_skip = c.ta.use_skip = false;
         for (auto &f : c.fin) f.use_skip = false;
     }
+    bool tuned_chi0 = false;  // tiles are tuned on the first tau point built at the plan's precision
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
+        // mixed precision: FP32 contractions for the tau points whose column of the cosine t->f
+        // matrix is small (the transform amplifies chi0(R, tau) errors by that column's entries)
+        cur = prec;
+        if (mixed) {
+            double amax = 0;
+            for (int w = 0; w < g.n; ++w) amax = std::max(amax, std::fabs(g.cos_t2f[(size_t)w * g.n + it]));
+            if (amax <= mixed_amax) cur = Prec::F32;
+        }
+        const bool lowp = cur != prec;
+        // low-precision tau points time under their own region names
+        auto rn = [lowp](const char *r) { return lowp ? std::string(r) + "_f32" : std::string(r); };
+        const double bsc = lowp ? 4.0 / ts : 1.0;  // algorithmic bytes scale with the element size
+        const void *Cfin_p = lowp ? d_Cfin32.get() : d_Cfin.get();
+        const void *CWb_p = lowp ? d_CWb32.get() : d_CWb.get();
+        if (lowp) HIP_CHECK(hipMemsetAsync(d_chi32.get(), 0, d_chi32.bytes(), stream));
         for (int s = 0; s < nspin; ++s) {
             build_gf(mf, tau, s);
             if (s == 0 && (screen_mode != 0 || collect)) screen_tau(it, tau);
-            char *chi_base = d_chi0s.as<char>() + (size_t)it * E_tot * ts;
-            const bool tune_now = autotune && it == 0 && s == 0;
+            char *chi_base = lowp ? d_chi32.as<char>() : d_chi0s.as<char>() + (size_t)it * E_tot * ts;
+            const bool tune_now = autotune && s == 0 && !lowp && !tuned_chi0;
             for (auto &c : chunks) {
-                prof.begin("chi0_gather", stream);
-                if (prec == Prec::F64)
+                prof.begin(rn("chi0_gather"), stream);
+                if (cur == Prec::F64)
                     launch_gather_cols<double>(d_Gpm.as<double>(), 2LL * NAO, d_Ghat.as<double>(), c.d_jobs.as<GatherJob>(),
                                                (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
                 else
                     launch_gather_cols<float>(d_Gpm.as<float>(), 2LL * NAO, d_Ghat.as<float>(), c.d_jobs.as<GatherJob>(),
                                               (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
-                prof.end("chi0_gather", stream, 0, 2.0 * c.ghat_elems * ts);
+                prof.end(rn("chi0_gather"), stream, 0, 2.0 * c.ghat_elems * (lowp ? 4 : ts));
                 if (tune_now && !c.wb.tuned) c.wb.tune(prec, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
-                prof.begin("chi0_W", stream);
-                gemm_run(prec, c.wb, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
-                prof.end("chi0_W", stream, c.wb.flops);
+                prof.begin(rn("chi0_W"), stream);
+                gemm_run(cur, c.wb, d_Ghat.get(), CWb_p, d_W.get(), 1.0, 0.0, stream);
+                prof.end(rn("chi0_W"), stream, c.wb.flops);
             }
             // T_A of chunk c + 1 (second stream, second T buffer) overlaps the contraction of chunk c
             const bool overlap = overlap_ta && !tune_now && chunks.size() > 1 && d_T2.bytes() >= t_max * ts;
@@ -561,19 +589,19 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     void *Tb = (ci % 2) ? d_T2.get() : d_T.get();
                     // the contraction two chunks back has finished reading this buffer
                     if (ci >= 2) HIP_CHECK(hipStreamWaitEvent(s2, ev_c[ci % 2], 0));
-                    prof.begin("chi0_TA", s2);
-                    gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), Tb, 1.0, 0.0, s2);
-                    prof.end("chi0_TA", s2, c.ta.flops, c.ta_bytes);
+                    prof.begin(rn("chi0_TA"), s2);
+                    gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), Tb, 1.0, 0.0, s2);
+                    prof.end(rn("chi0_TA"), s2, c.ta.flops, c.ta_bytes * bsc);
                     HIP_CHECK(hipEventRecord(ev_ta[ci % 2], s2));
-                    prof.begin("chi0_X", stream);
-                    gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
-                    prof.end("chi0_X", stream, c.xb.flops);
+                    prof.begin(rn("chi0_X"), stream);
+                    gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                    prof.end(rn("chi0_X"), stream, c.xb.flops);
                     HIP_CHECK(hipStreamWaitEvent(stream, ev_ta[ci % 2], 0));
                     for (int k = 0; k < 4; ++k) {
                         if (c.fin[k].total_tiles == 0) continue;
-                        prof.begin("chi0_contract", stream);
-                        gemm_run(prec, c.fin[k], d_Cfin.get(), Tb, chi_base, 1.0, 1.0, stream);
-                        prof.end("chi0_contract", stream, c.fin[k].flops, c.fin_bytes[k]);
+                        prof.begin(rn("chi0_contract"), stream);
+                        gemm_run(cur, c.fin[k], Cfin_p, Tb, chi_base, 1.0, 1.0, stream);
+                        prof.end(rn("chi0_contract"), stream, c.fin[k].flops, c.fin_bytes[k] * bsc);
                     }
                     HIP_CHECK(hipEventRecord(ev_c[ci % 2], stream));
                 }
@@ -590,21 +618,25 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                     for (auto &f : c.fin) f.tune(prec, d_Cfin.get(), d_T.get(), chi_base, 1.0, 0.0, stream);
                 }
-                prof.begin("chi0_X", stream);
-                gemm_run(prec, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
-                prof.end("chi0_X", stream, c.xb.flops);
-                prof.begin("chi0_TA", stream);
-                gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                prof.end("chi0_TA", stream, c.ta.flops, c.ta_bytes);
+                prof.begin(rn("chi0_X"), stream);
+                gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                prof.end(rn("chi0_X"), stream, c.xb.flops);
+                prof.begin(rn("chi0_TA"), stream);
+                gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                prof.end(rn("chi0_TA"), stream, c.ta.flops, c.ta_bytes * bsc);
                 // one timing region per launch (per block-shape class)
                 for (int k = 0; k < 4; ++k) {
                     if (c.fin[k].total_tiles == 0) continue;
-                    prof.begin("chi0_contract", stream);
-                    gemm_run(prec, c.fin[k], d_Cfin.get(), d_T.get(), chi_base, 1.0, 1.0, stream);
-                    prof.end("chi0_contract", stream, c.fin[k].flops, c.fin_bytes[k]);
+                    prof.begin(rn("chi0_contract"), stream);
+                    gemm_run(cur, c.fin[k], Cfin_p, d_T.get(), chi_base, 1.0, 1.0, stream);
+                    prof.end(rn("chi0_contract"), stream, c.fin[k].flops, c.fin_bytes[k] * bsc);
                 }
             }
         }
+        tuned_chi0 = tuned_chi0 || (autotune && !lowp);
+        // the FP32 result of a low-precision tau point goes into the FP64 chi0(R, tau) store
+        if (lowp)
+            launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (size_t)it * E_tot, E_tot, stream);
     }
     HIP_CHECK(hipStreamSynchronize(stream));
     ntau_done = ntau;

@@ -65,6 +65,12 @@ struct RpaEngine {
 
     // ---- device data ----
     size_t ts = 8;  // storage element size (8 fp64, 4 fp32)
+    // mixed precision (FP64 plan): tau points whose cosine t->f column has no entry above
+    // mixed_amax run their contractions in FP32 (operands converted once, result stored FP64)
+    bool mixed = false;
+    double mixed_amax = 1e3;
+    Prec cur = Prec::F64;  // precision of the tau point being built
+    DevBuf d_Cfin32, d_CWb32, d_chi32;
     DevBuf d_Cfin, d_CWb, d_ZZ, d_Zs, d_Mst, d_Gpm, d_Ghat, d_W, d_T, d_T2, d_chi0s;
     DevBuf d_wre, d_wim, d_eps, d_wg, d_cs, d_sn, d_gcol;
     hipStream_t stream = 0;

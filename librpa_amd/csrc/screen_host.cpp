@@ -76,7 +76,7 @@ void RpaEngine::screen_tau(int it, double tau) {
     if (nout == 0) return;
     auto &prof = profiler();
     prof.begin("chi0_screen", stream);
-    if (prec == Prec::F64)
+    if (cur == Prec::F64)
         launch_block_norms<double>(d_Gpm.as<double>(), 2LL * NAO, d_gblocks.as<BlockRef>(), 2 * nout,
                                    d_gnorm.as<double>(), stream);
     else

@@ -130,3 +130,29 @@ def test_cell16_reduced_precision_report(system, golden, fp64, prec, tol_chi):
          f"dEc/atom {(ec - d['ec_ha']) / system.natom:.3e} Ha")
     assert err < tol_chi
     assert abs(ec - d["ec_ha"]) / system.natom < 1e-5
+
+
+@pytest.mark.gpu
+def test_cell16_mixed_precision(system, golden, fp64):
+    # MIXED: FP64 at the tau points whose cosine-transform column has an entry above 1e3, FP32
+    # at the others.  The FP64 points must reproduce the FP64 run, the FP32 ones carry FP32
+    # arithmetic errors, and Ec holds the 1e-5 Ha/atom bar.
+    d, _ = golden
+    _say("cell16 mixed precision")
+    h, o, ec, _ = _run(system, L.MIXED)
+    grid = h.get_tfgrids(o)
+    amax = np.abs(grid["cos_t2f"]).max(axis=0)
+    low = [t for t in range(system.nfreq) if amax[t] <= 1e3]
+    _say(f"  FP32 tau points {low}; Ec {ec!r} oracle {d['ec_ha']!r}; dEc/atom {(ec - d['ec_ha']) / system.natom:.3e} Ha")
+    assert 0 < len(low) < system.nfreq
+    for it in sorted(set(CHECK_TAU) | set(REDUCED_TAU)):
+        ref = fp64["blocks"][it]
+        scale = max(np.abs(b).max() for b in ref.values())
+        got = _blocks(h, system, it)
+        err = max(np.abs(got[k] - ref[k]).max() for k in ref) / scale
+        _say(f"  tau {it} ({'fp32' if it in low else 'fp64'}): max rel err {err:.2e}")
+        if it not in low:
+            assert err <= 1e-12
+        elif it in REDUCED_TAU:
+            assert err < 1e-4
+    assert abs(ec - d["ec_ha"]) / system.natom < 1e-5
