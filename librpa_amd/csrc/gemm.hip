@@ -90,11 +90,13 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
                                                     const unsigned char *__restrict__ skip, int skip_zero) {
     using T = typename ModeTraits<MODE>::T;
     using V = typename ModeTraits<MODE>::V;
-    // the chi0 contraction (VAR 3, K ~ 25k) runs FP64 with a 32-deep K slice: half the barriers per flop
-    constexpr int BK = (VAR == 3 && MODE == 0) ? 32 : ModeTraits<MODE>::BK;
+    constexpr int NT = WM * WN * 64;
+    // the chi0 contraction (VAR 3, K ~ 25k) runs with a 32-deep K slice: half the barriers per
+    // flop; T_A (VAR 4, K = 2 n_basis(J), e.g. 40) with an 8-deep one, which pads K by at most 7
+    constexpr bool bk8 = VAR == 4 && MODE != 2 && (BM * 8) % NT == 0 && (BN * 8) % NT == 0;
+    constexpr int BK = (VAR == 3 && MODE != 2) ? 32 : bk8 ? 8 : ModeTraits<MODE>::BK;
     constexpr int PAD = ModeTraits<MODE>::PAD;
     constexpr int LDK = BK + PAD;
-    constexpr int NT = WM * WN * 64;
     static_assert(WM * WN == 4 || WM * WN == 8, "4 or 8 waves");
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
