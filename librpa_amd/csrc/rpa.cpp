@@ -638,6 +638,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         if (lowp)
             launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (size_t)it * E_tot, E_tot, stream);
     }
+    cur = prec;  // later G builds (the self-energy's) run at the plan's precision
     HIP_CHECK(hipStreamSynchronize(stream));
     ntau_done = ntau;
     t_chi0 = now_s() - t0;

@@ -122,6 +122,35 @@ def test_reduced_precision_cell2():
         assert np.isfinite(ec)
 
 
+@pytest.mark.parametrize("case", ["cell2_k8", "cell2_k8_spin"])
+def test_mixed_precision_against_fp64(case):
+    """MIXED: FP64 contractions at the tau points whose cosine-transform column has an entry
+    above 1e3, FP32 at the others (on cell2: tau 8..15).  FP64 points reproduce the FP64 path
+    (1e-12), FP32 points carry FP32 errors (2e-5, the FP32 bar above), and |dEc| <= 1e-5
+    Ha/atom.  The spin-polarised case accumulates both spins into the FP32 staging buffer."""
+    _say(f"mixed precision case={case}")
+    s = CASES[case]()
+    h64, _, ec64, _, grid = _run_lib(s)
+    h, _, ec, contrib, _ = _run_lib(s, L.MIXED)
+    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= 1e3)
+    assert 0 < len(low) < s.nfreq
+    e64 = e32 = 0.0
+    for it in range(s.nfreq):
+        for I, J in ((0, 0), (0, 1)):
+            r = h64.get_chi0s_block(it, I, J, (0, 0, 0), int(s.nabf[I]), int(s.nabf[J]))
+            g = h.get_chi0s_block(it, I, J, (0, 0, 0), int(s.nabf[I]), int(s.nabf[J]))
+            e = np.abs(g - r).max() / np.abs(r).max()
+            if it in low:
+                e32 = max(e32, e)
+            else:
+                e64 = max(e64, e)
+    _say(f"  FP32 tau {sorted(low)}: rel err fp64 points {e64:.1e}, fp32 points {e32:.1e}; "
+         f"Ec {ec!r} fp64 {ec64!r} dEc/atom {(ec - ec64) / s.natom:.3e}")
+    assert e64 <= 1e-12 and e32 < 2e-5
+    assert abs(ec - ec64) / s.natom < 1e-5
+    assert np.isfinite(contrib).all()
+
+
 @pytest.mark.parametrize("case,ibz", [("tiny_k3_odd", [0, 1, 1]), ("cell2_k8", [0, 0, 2, 2, 4, 4, 4, 7])])
 def test_ibz_mapping(case, ibz):
     # Ec over the IBZ list of an irreducible-k mapping, each k weighted by its star

@@ -85,11 +85,12 @@ def test_h2o_g0w0_library_matches_reference_table():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec_name,bar_mev", [("fp32", 1.0), ("bf16x3", 25.0)])
+@pytest.mark.parametrize("prec_name,bar_mev", [("fp32", 1.0), ("bf16x3", 25.0), ("mixed", 1.0)])
 def test_h2o_g0w0_reduced_precision_qp_error(prec_name, bar_mev):
     # chi0 and Sigma_c contractions in FP32 / BF16x3 (W inversion, quadratures, Pade and the QP
-    # solve stay FP64): quasi-particle energies of all 24 states against the FP64 library path,
-    # which matches the reference table above
+    # solve stay FP64), or chi0 in FP32 at the tau points of small cosine-transform weight only
+    # (mixed): quasi-particle energies of all 24 states against the FP64 library path, which
+    # matches the reference table above
     import librpa_amd as L
     ds, vc, vxc = _inputs()
     vexx = np.array([r[4] / H for r in EXP["rows"]])
@@ -97,7 +98,7 @@ def test_h2o_g0w0_reduced_precision_qp_error(prec_name, bar_mev):
     for name in ("fp64", prec_name):
         h = L.Handler()
         driver.push(h, ds, L.LIBRI, vq_cut=vc)
-        h.set_precision(L.PREC[name], L.PREC[name])
+        h.set_precision(L.PREC[name], L.FP64 if name == "mixed" else L.PREC[name])
         o = L.Options(nfreq=16, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI,
                       n_params_anacon=EXP["options"]["n_params_anacon"])
         sc = h.get_g0w0_sigc_kgrid(o, 1, [0], 0, 24, vxc[0, 0], vexx)
