@@ -87,9 +87,17 @@ struct GemmBatch {
 
     bool tuned = false;
     float tuned_ms = 0;    // time of the chosen tile shape in the tuning run
+    // tuned tile shape per precision (index Prec), -1 where not tuned: a caller that launches one
+    // batch at two precisions re-finalizes with use_tiles before switching
+    int cfg_by_prec[3] = {-1, -1, -1};
 
     void add(const GemmProb &p) { probs.push_back(p); }
     void finalize(int cfg_, hipStream_t s = 0);
+    // switch to the shape tuned for prec (if any and different from the current one)
+    void use_tiles(Prec prec, hipStream_t s) {
+        const int c = cfg_by_prec[(int)prec];
+        if (c >= 0 && c != cfg) finalize(c, s);
+    }
     bool empty() const { return probs.empty(); }
     // Autotuner: time one launch of every tile shape this batch can use (after a warm-up
     // launch) on the real operands and keep the fastest.  The launches must be harmless

@@ -452,6 +452,7 @@ int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alp
             finalize(it->second.first, s);
             tuned = true;
             tuned_ms = it->second.second;
+            cfg_by_prec[(int)prec] = cfg;
             return cfg;
         }
     }
@@ -491,6 +492,7 @@ int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alp
     use_skip = was_skip;
     tuned = true;
     tuned_ms = best;
+    cfg_by_prec[(int)prec] = bc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         tune_cache()[key] = {bc, best};

@@ -542,7 +542,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         c.xb.use_skip = c.ta.use_skip = false;
         for (auto &f : c.fin) f.use_skip = false;
     }
-    bool tuned_chi0 = false;  // tiles are tuned on the first tau point built at the plan's precision
+    bool tuned_chi0[2] = {false, false};  // [plan precision, FP32 of the mixed mode]
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
         // mixed precision: FP32 contractions for the tau points whose column of the cosine t->f
@@ -560,11 +560,19 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         const void *Cfin_p = lowp ? d_Cfin32.get() : d_Cfin.get();
         const void *CWb_p = lowp ? d_CWb32.get() : d_CWb.get();
         if (lowp) HIP_CHECK(hipMemsetAsync(d_chi32.get(), 0, d_chi32.bytes(), stream));
+        if (mixed)
+            for (auto &c : chunks) {
+                c.wb.use_tiles(cur, stream);
+                c.xb.use_tiles(cur, stream);
+                c.ta.use_tiles(cur, stream);
+                for (auto &f : c.fin) f.use_tiles(cur, stream);
+            }
         for (int s = 0; s < nspin; ++s) {
             build_gf(mf, tau, s);
             if (s == 0 && (screen_mode != 0 || collect)) screen_tau(it, tau);
             char *chi_base = lowp ? d_chi32.as<char>() : d_chi0s.as<char>() + (size_t)it * E_tot * ts;
-            const bool tune_now = autotune && s == 0 && !lowp && !tuned_chi0;
+            // tiles are tuned per precision on the first tau point built at it
+            const bool tune_now = autotune && s == 0 && !tuned_chi0[lowp];
             for (auto &c : chunks) {
                 prof.begin(rn("chi0_gather"), stream);
                 if (cur == Prec::F64)
@@ -574,7 +582,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     launch_gather_cols<float>(d_Gpm.as<float>(), 2LL * NAO, d_Ghat.as<float>(), c.d_jobs.as<GatherJob>(),
                                               (int)c.jobs.size(), c.d_colsrc.as<long long>(), NAO, c.max_gcol, stream);
                 prof.end(rn("chi0_gather"), stream, 0, 2.0 * c.ghat_elems * (lowp ? 4 : ts));
-                if (tune_now && !c.wb.tuned) c.wb.tune(prec, d_Ghat.get(), d_CWb.get(), d_W.get(), 1.0, 0.0, stream);
+                if (tune_now && c.wb.cfg_by_prec[(int)cur] < 0) c.wb.tune(cur, d_Ghat.get(), CWb_p, d_W.get(), 1.0, 0.0, stream);
                 prof.begin(rn("chi0_W"), stream);
                 gemm_run(cur, c.wb, d_Ghat.get(), CWb_p, d_W.get(), 1.0, 0.0, stream);
                 prof.end(rn("chi0_W"), stream, c.wb.flops);
@@ -611,12 +619,12 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 continue;
             }
             for (auto &c : chunks) {
-                if (tune_now && !c.ta.tuned) {
+                if (tune_now && c.ta.cfg_by_prec[(int)cur] < 0) {
                     // TA overwrites T; the contraction trials overwrite chi0s with beta = 0 before
                     // the X term below resets it for this tau
-                    c.ta.tune(prec, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                    gemm_run(prec, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                    for (auto &f : c.fin) f.tune(prec, d_Cfin.get(), d_T.get(), chi_base, 1.0, 0.0, stream);
+                    c.ta.tune(cur, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    for (auto &f : c.fin) f.tune(cur, Cfin_p, d_T.get(), chi_base, 1.0, 0.0, stream);
                 }
                 prof.begin(rn("chi0_X"), stream);
                 gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
@@ -633,7 +641,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 }
             }
         }
-        tuned_chi0 = tuned_chi0 || (autotune && !lowp);
+        tuned_chi0[lowp] = tuned_chi0[lowp] || autotune;
         // the FP32 result of a low-precision tau point goes into the FP64 chi0(R, tau) store
         if (lowp)
             launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (size_t)it * E_tot, E_tot, stream);
