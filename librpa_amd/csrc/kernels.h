@@ -49,6 +49,18 @@ template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s)
 template <class T> void launch_zero_upper(T *A, int n, int lda, hipStream_t s);
 template <class TS, class TD> void launch_convert(const TS *src, TD *dst, long long n, hipStream_t s);
 void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStream_t s);
+
+// One upper atom-pair block V_IJ(q) (I <= J) of the Coulomb matrix, complex, na_I x na_J
+// row-major at element offset off of the block store; r0 / c0 are the ABF offsets of I and J.
+struct VBlock {
+    long long off;
+    int r0, c0, na, nb, diag;
+};
+// Dense real V(q) (n2 = n, Gamma) or its real embedding [[Re, -Im], [Im, Re]] (n2 = 2n) from the
+// upper blocks, Hermitian fill; V must be zeroed first.  Every dense entry is written by at most
+// one (block, element), so the blocks run in parallel.
+void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, int n, int n2, int embed,
+                      hipStream_t s);
 void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s);
 void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s);
 

@@ -257,6 +257,37 @@ void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStr
     HIP_CHECK(hipGetLastError());
 }
 
+__global__ void v_scatter_kernel(const double2 *__restrict__ blk, const VBlock *__restrict__ desc, double *V,
+                                 long long n, long long n2, int embed) {
+    const VBlock b = desc[blockIdx.x];
+    const int tot = b.na * b.nb;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int mu = e / b.nb, nu = e % b.nb;
+        const double2 v = blk[b.off + e];
+        const long long r = b.r0 + mu, c = b.c0 + nu;
+        V[r * n2 + c] = v.x;
+        if (!b.diag) V[c * n2 + r] = v.x;
+        if (embed) {
+            V[(n + r) * n2 + n + c] = v.x;
+            V[(n + r) * n2 + c] = v.y;
+            V[r * n2 + n + c] = -v.y;
+            if (!b.diag) {
+                V[(n + c) * n2 + n + r] = v.x;
+                V[(n + c) * n2 + r] = -v.y;
+                V[c * n2 + n + r] = v.y;
+            }
+        }
+    }
+}
+
+void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, int n, int n2, int embed,
+                      hipStream_t s) {
+    if (nblk == 0) return;
+    hipLaunchKernelGGL(v_scatter_kernel, dim3(nblk), dim3(256), 0, s, reinterpret_cast<const double2 *>(blk), d, V,
+                       (long long)n, (long long)n2, embed);
+    HIP_CHECK(hipGetLastError());
+}
+
 __global__ void trace_kernel(const double *__restrict__ A, int n, int lda, double *acc) {
     __shared__ double red[256];
     double v = 0;

@@ -331,8 +331,6 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                                      na[O.J], O.eoff, na[O.J]));
             c.fin_owner[cls].push_back(o);
         }
-        const bool bf = prec == Prec::BF16X3;
-        (void)bf;
         c.wb.finalize(big_tile(prec), stream);
         c.xb.trans_a = true;
         c.xb.finalize(T64x64, stream);
@@ -698,7 +696,26 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     CholWork cw;
     std::vector<double> gp;
     DevBuf dgp;
-    std::vector<double> Vh(nn);
+    if (v_ibz != ibz) {
+        // stage the input's V(q) blocks once per plan (the engine is rebuilt when inputs change)
+        std::vector<VBlock> desc;
+        std::vector<cplx> store;
+        vdesc_q.assign(1, 0);
+        for (int iq = 0; iq < nq; ++iq) {
+            for (const auto &kv : ds.vq) {
+                const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
+                if (ik != ibz[iq] || I > J) continue;
+                if (kv.second.size() != (size_t)na[I] * na[J]) LRA_THROW("Coulomb block of the wrong size");
+                desc.push_back({(long long)store.size(), (int)abf[I], (int)abf[J], na[I], na[J], I == J ? 1 : 0});
+                store.insert(store.end(), kv.second.begin(), kv.second.end());
+            }
+            vdesc_q.push_back((int)desc.size());
+        }
+        d_vblk = upload(store, stream);
+        d_vdesc = upload(desc, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        v_ibz = ibz;
+    }
     // (chunk size, order, precision) -> (X = D L, M = I - L^T X), kept with their tuned tiles
     auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
         const auto key = std::make_tuple(cnt, n2, (int)ec_prec);
@@ -726,34 +743,12 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     };
     for (int iq = 0; iq < nq; ++iq) {
         const int kq = ibz[iq];
-        // dense V(q) from upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
-        std::fill(Vh.begin(), Vh.end(), 0.0);
-        for (const auto &kv : ds.vq) {
-            const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
-            if (ik != kq || I > J) continue;
-            const auto &blk = kv.second;
-            for (int mu = 0; mu < na[I]; ++mu)
-                for (int nu = 0; nu < na[J]; ++nu) {
-                    const cplx v = blk[(size_t)mu * na[J] + nu];
-                    const size_t r = abf[I] + mu, c = abf[J] + nu;
-                    if (!embed) {
-                        Vh[r * n2 + c] = v.real();
-                        if (I != J) Vh[c * n2 + r] = v.real();
-                    } else {
-                        Vh[r * n2 + c] = v.real();
-                        Vh[(n + r) * n2 + n + c] = v.real();
-                        Vh[(n + r) * n2 + c] = v.imag();
-                        Vh[r * n2 + n + c] = -v.imag();
-                        if (I != J) {
-                            Vh[c * n2 + r] = v.real();
-                            Vh[(n + c) * n2 + n + r] = v.real();
-                            Vh[(n + c) * n2 + r] = -v.imag();
-                            Vh[c * n2 + n + r] = v.imag();
-                        }
-                    }
-                }
-        }
-        HIP_CHECK(hipMemcpyAsync(dV.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, stream));
+        // dense V(q) from the staged upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
+        prof.begin("ec_v_fill", stream);
+        HIP_CHECK(hipMemsetAsync(dV.get(), 0, nn * 8, stream));
+        launch_v_scatter(d_vblk.as<double>(), d_vdesc.as<VBlock>() + vdesc_q[iq], vdesc_q[iq + 1] - vdesc_q[iq],
+                         dV.as<double>(), n, n2, embed ? 1 : 0, stream);
+        prof.end("ec_v_fill", stream, 0, (double)nn * 8);
         prof.begin("ec_chol_v", stream);
         cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream);
         prof.end("ec_chol_v", stream, (double)n2 * n2 * n2 / 3.0);

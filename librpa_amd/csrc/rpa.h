@@ -91,6 +91,10 @@ struct RpaEngine {
     std::vector<long long> pair_eoff;  // [pair][r]
     long long n_pair_elems = 0;
     DevBuf d_pairs, d_pair_eoff;
+    // the input V(q) upper atom-pair blocks of the irreducible q points, staged on the device at
+    // the first Ec call (block store, descriptors, first descriptor of each q)
+    DevBuf d_vblk, d_vdesc;
+    std::vector<int> vdesc_q, v_ibz;
 
     // counters
     double flops_chi0_alg = 0;   // canonical count per spin, all tau
