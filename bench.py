@@ -141,20 +141,25 @@ def cpu_baseline(sys_, grid, nfreq):
     Gn = rpa.gf_R(sys_, -tau, Rs)
     t_g = time.perf_counter() - t0
     J = sys_.natom // 2
-    nI = 3
     t0 = time.perf_counter()
     rpa.chi0s_factorized(sys_, cs, Gp, Gn, [(0, J)], Rs)
     t_one = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    rpa.chi0s_factorized(sys_, cs, Gp, Gn, [(I, J) for I in range(nI)], Rs)
-    t_three = time.perf_counter() - t0
-    t_out = max((t_three - t_one) / (nI - 1), 1e-9)
+    # output blocks of one J (one W build): the block count doubles until a run takes 6 s
+    nI = 2
+    while True:
+        t0 = time.perf_counter()
+        rpa.chi0s_factorized(sys_, cs, Gp, Gn, [(I, J) for I in range(nI)], Rs)
+        t_n = time.perf_counter() - t0
+        if t_n >= 6.0 or nI >= sys_.natom:
+            break
+        nI = min(2 * nI, sys_.natom)
+    t_out = max((t_n - t_one) / (nI - 1), 1e-9)
     t_w = max(t_one - t_out, 0.0)
     n_out = sys_.natom * (sys_.natom + 1) // 2 * len(Rs)
     chi0_per_tau = t_g + sys_.natom * t_w + n_out * t_out
-    # Ec: one omega at a quarter-size matrix, scaled by n^3
+    # Ec: one omega at a half-size matrix, scaled by n^3
     n = int(sys_.nabf.sum())
-    m = max(n // 4, 64)
+    m = max(n // 2, 64)
     rng = np.random.default_rng(0)
     b = rng.standard_normal((m, m))
     V = b @ b.T / m + 1e-3 * np.eye(m)
