@@ -97,20 +97,28 @@ def test_cell16_fp64_matches_oracle(system, golden, fp64):
 
 
 @pytest.mark.gpu
-def test_cell16_small_chunks_match_oracle(system, golden, fp64, monkeypatch):
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_cell16_small_chunks_match_oracle(system, golden, fp64, monkeypatch, prec):
     # chunks of a few atoms each: the T_A / contraction overlap across chunks and the chunk
-    # bookkeeping must reproduce the single-chunk result
+    # bookkeeping must reproduce the single-chunk result (mixed: at its FP64 tau points; the
+    # FP32 ones hold the FP32 bar)
     d, _ = golden
     monkeypatch.setenv("LIBRPA_AMD_CHUNK_BYTES", "2e8")
     monkeypatch.setenv("LIBRPA_AMD_OVERLAP", "1")
-    _say("cell16 fp64 run with small chunks")
-    h, o, ec, contrib = _run(system, L.FP64)
+    _say(f"cell16 prec={prec} run with small chunks")
+    h, o, ec, contrib = _run(system, prec)
     _say(f"  Ec {ec!r} oracle {d['ec_ha']!r} single-chunk {fp64['ec']!r}")
-    assert abs(ec - d["ec_ha"]) < 1e-8
-    for it in (0, 12):
+    if prec == L.FP64:
+        assert abs(ec - d["ec_ha"]) < 1e-8
+    else:
+        assert abs(ec - d["ec_ha"]) / system.natom < 1e-5
+    grid = h.get_tfgrids(o)
+    for it in (0, 12, 17):
+        low = prec == L.MIXED and np.abs(grid["cos_t2f"][:, it]).max() <= 1e3
         ref = fp64["blocks"][it]
         got = _blocks(h, system, it)
-        assert max(np.abs(got[k] - ref[k]).max() for k in ref) <= 1e-12 * max(np.abs(b).max() for b in ref.values())
+        err = max(np.abs(got[k] - ref[k]).max() for k in ref) / max(np.abs(b).max() for b in ref.values())
+        assert err <= (1e-4 if low else 1e-12), (it, err)
 
 
 @pytest.mark.gpu

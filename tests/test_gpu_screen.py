@@ -13,9 +13,10 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _run(s, keep=False, mode=L.SCREEN_OFF, thr=1e-8, model=None):
+def _run(s, keep=False, mode=L.SCREEN_OFF, thr=1e-8, model=None, prec=L.FP64):
     h = L.Handler()
     L.load_system(h, s)
+    h.set_precision(prec, L.FP64)
     h.set_keep_chi0(keep)
     h.set_screening(mode, thr, model)
     o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
@@ -112,6 +113,36 @@ def test_bound_screening_is_exact_on_kept_blocks(lib):
                 else:
                     assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
     assert nz == st["skipped"]
+
+
+@pytest.mark.gpu
+def test_bound_screening_with_mixed_precision(lib):
+    # the mixed mode's FP32 tau points take their G-block norms from the FP32 G panel; screened
+    # and unscreened mixed builds must agree on every kept block, and no skipped block may
+    # exceed the threshold (up to FP32 rounding of the norms)
+    _say("bound screening cell2, mixed precision")
+    s = synth.cell2()
+    h0, o, ec0 = _run(s, keep=True, prec=L.MIXED)
+    h64, _, _ = _run(s, keep=True)
+    _, t = h64.get_screen_samples()
+    thr = float(np.exp(np.median(t)))
+    h1, _, ec1 = _run(s, keep=True, mode=L.SCREEN_BOUND, thr=thr, prec=L.MIXED)
+    st = h1.get_screen_stats()
+    _say(f"  threshold {thr:.2e}: skipped {st['skipped']:.0f} of {st['blocks']:.0f} blocks, dEc {ec1 - ec0:.3e}")
+    assert st["skipped"] > 0
+    nz = 0
+    for it in range(s.nfreq):
+        for (I, J) in rpa.upper_pairs(s.natom):
+            for R in rpa.r_grid(s.period):
+                a = h0.get_chi0s_block(it, I, J, R, 60, 60)
+                b = h1.get_chi0s_block(it, I, J, R, 60, 60)
+                if not b.any():
+                    nz += 1
+                    assert np.linalg.norm(a) < thr * (1 + 1e-5)
+                else:
+                    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+    assert nz == st["skipped"]
+    assert abs(ec1 - ec0) / s.natom < 1e-5
 
 
 @pytest.mark.gpu
