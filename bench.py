@@ -157,9 +157,9 @@ def cpu_baseline(sys_, grid, nfreq):
     t_w = max(t_one - t_out, 0.0)
     n_out = sys_.natom * (sys_.natom + 1) // 2 * len(Rs)
     chi0_per_tau = t_g + sys_.natom * t_w + n_out * t_out
-    # Ec: one omega at a half-size matrix, scaled by n^3
+    # Ec: one omega at the full matrix size (the n^3 scaling below is then 1)
     n = int(sys_.nabf.sum())
-    m = max(n // 2, 64)
+    m = n
     rng = np.random.default_rng(0)
     b = rng.standard_normal((m, m))
     V = b @ b.T / m + 1e-3 * np.eye(m)
