@@ -1,6 +1,6 @@
 """Block-norm screener on the device: MLP kernels against the oracle, the features the chi0
 build emits, exactness of screened builds, and what the learned screener gains over the
-norm-product bound (the unscreened FP64 build is the judge of dEc)."""
+norm-product bound (dEc is measured against the unscreened FP64 build)."""
 import numpy as np
 import pytest
 
