@@ -446,40 +446,45 @@ void ns_sqrt(double *A, int n, hipStream_t st) {
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
-// dense (embedded when complex) Hermitian matrix of atom-pair blocks I <= J for k point kq
-void dense_coulomb(const RpaEngine &R0, const std::map<std::tuple<int, int, int>, std::vector<cplx>> &store, int kq,
-                   bool embed, bool real_part, std::vector<double> &Vh, double &imax, double &vmax) {
-    const int n = R0.NABF, n2 = embed ? 2 * n : n;
-    Vh.assign((size_t)n2 * n2, 0.0);
-    imax = vmax = 0;
-    for (const auto &kv : store) {
-        const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
-        if (ik != kq || I > J) continue;
-        const auto &blk = kv.second;
-        for (int mu = 0; mu < R0.na[I]; ++mu)
-            for (int nu = 0; nu < R0.na[J]; ++nu) {
-                const cplx v = blk[(size_t)mu * R0.na[J] + nu];
-                vmax = std::max(vmax, std::abs(v));
-                imax = std::max(imax, std::fabs(v.imag()));
-                const size_t r = R0.abf[I] + mu, c = R0.abf[J] + nu;
-                const double vi = real_part ? 0.0 : v.imag();
-                if (!embed) {
-                    Vh[r * n2 + c] = v.real();
-                    if (I != J) Vh[c * n2 + r] = v.real();
-                } else {
-                    Vh[r * n2 + c] = v.real();
-                    Vh[(n + r) * n2 + n + c] = v.real();
-                    Vh[(n + r) * n2 + c] = vi;
-                    Vh[r * n2 + n + c] = -vi;
-                    if (I != J) {
-                        Vh[c * n2 + r] = v.real();
-                        Vh[(n + c) * n2 + n + r] = v.real();
-                        Vh[(n + c) * n2 + r] = -vi;
-                        Vh[c * n2 + n + r] = vi;
-                    }
-                }
+// The upper atom-pair blocks (I <= J) of a Coulomb input for every k, staged on the device for
+// launch_v_scatter, with the largest |V| and |Im V| of each k.
+struct CoulombStage {
+    DevBuf blk, desc;
+    std::vector<int> first;  // descriptors of k: [first[k], first[k + 1])
+    std::vector<double> vmax, imax;
+};
+void stage_coulomb(const RpaEngine &R0, const std::map<std::tuple<int, int, int>, std::vector<cplx>> &store, int nk,
+                   CoulombStage &out, hipStream_t st) {
+    std::vector<VBlock> desc;
+    std::vector<cplx> vals;
+    out.first.assign(1, 0);
+    out.vmax.assign(nk, 0.0);
+    out.imax.assign(nk, 0.0);
+    for (int k = 0; k < nk; ++k) {
+        for (const auto &kv : store) {
+            const int ik = std::get<0>(kv.first), I = std::get<1>(kv.first), J = std::get<2>(kv.first);
+            if (ik != k || I > J) continue;
+            if (kv.second.size() != (size_t)R0.na[I] * R0.na[J]) LRA_THROW("Coulomb block of the wrong size");
+            for (const cplx &v : kv.second) {
+                out.vmax[k] = std::max(out.vmax[k], std::abs(v));
+                out.imax[k] = std::max(out.imax[k], std::fabs(v.imag()));
             }
+            desc.push_back({(long long)vals.size(), (int)R0.abf[I], (int)R0.abf[J], R0.na[I], R0.na[J], I == J ? 1 : 0});
+            vals.insert(vals.end(), kv.second.begin(), kv.second.end());
+        }
+        out.first.push_back((int)desc.size());
     }
+    out.blk = upload(vals, st);
+    out.desc = upload(desc, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// dense (embedded when complex) Hermitian matrix of k point kq into V (device)
+void dense_coulomb(const CoulombStage &cs, int kq, int n, int n2, bool embed, bool real_part, double *V,
+                   hipStream_t st) {
+    HIP_CHECK(hipMemsetAsync(V, 0, sizeof(double) * (size_t)n2 * n2, st));
+    launch_v_scatter(cs.blk.as<double>(), cs.desc.as<VBlock>() + cs.first[kq], cs.first[kq + 1] - cs.first[kq], V, n,
+                     n2, embed ? 1 : 0, real_part ? 1 : 0, st);
 }
 }  // namespace
 
@@ -523,7 +528,9 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), st));
     CholWork cw;
     DevBuf dSc;
-    std::vector<double> Vh, Vch;
+    CoulombStage vst, vcst;
+    stage_coulomb(R0, ds.vq, nk, vst, st);
+    if (!same_cut) stage_coulomb(R0, ds.vq_cut, nk, vcst, st);
     std::map<int, std::pair<GemmBatch, GemmBatch>> batches;
     auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
         auto it = batches.find(cnt);
@@ -551,30 +558,27 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     for (int iq = 0; iq < nk; ++iq) {
         const double *kf = &ds.kfrac[(size_t)iq * 3];
         const bool gamma = std::fabs(kf[0]) < 1e-12 && std::fabs(kf[1]) < 1e-12 && std::fabs(kf[2]) < 1e-12;
-        double vmax = 0, imax = 0;
+        const double vmax = vst.vmax[iq], imax = vst.imax[iq];
         // v^{1/2} from Re v at q = 0 (power_hemat_blacs_real, epsilon.cpp:2019-2024)
-        dense_coulomb(R0, ds.vq, iq, embed, gamma, Vh, imax, vmax);
         if (same_cut) {
             // with v_c = v the Cholesky identity needs one matrix for both roots: real v(0)
             if (gamma && imax > 1e-12 * std::max(vmax, 1e-300))
                 LRA_THROW("W at q = 0 with v_c = v needs a real Coulomb matrix (Im v(0) != 0)");
-            HIP_CHECK(hipMemcpyAsync(dV0.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, st));
+            dense_coulomb(vst, iq, n, n2, embed, gamma, dV0.as<double>(), st);
             HIP_CHECK(hipMemcpyAsync(dL.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
             prof.begin("gw_w_chol_v", st);
             cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st);
             prof.end("gw_w_chol_v", st, (double)n2 * n2 * n2 / 3.0);
         } else {
             // dL <- v^{1/2}; dV0 <- v_c, dSc <- v_c^{1/2} (complex v_c kept, as the reference does)
-            double ic = 0, vc = 0;
-            dense_coulomb(R0, ds.vq_cut, iq, embed, false, Vch, ic, vc);
-            if (!embed && ic > 1e-12 * std::max(vc, 1e-300))
+            if (!embed && vcst.imax[iq] > 1e-12 * std::max(vcst.vmax[iq], 1e-300))
                 LRA_THROW("a complex truncated Coulomb matrix at Gamma-only sampling is outside this build");
             prof.begin("gw_w_sqrt_v", st);
-            HIP_CHECK(hipMemcpyAsync(dL.get(), Vh.data(), nn * 8, hipMemcpyHostToDevice, st));
+            dense_coulomb(vst, iq, n, n2, embed, gamma, dL.as<double>(), st);
             ns_sqrt(dL.as<double>(), n2, st);
-            HIP_CHECK(hipMemcpyAsync(dV0.get(), Vch.data(), nn * 8, hipMemcpyHostToDevice, st));
+            dense_coulomb(vcst, iq, n, n2, embed, false, dV0.as<double>(), st);
             dSc.reserve(nn * 8);
-            HIP_CHECK(hipMemcpyAsync(dSc.get(), Vch.data(), nn * 8, hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipMemcpyAsync(dSc.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
             ns_sqrt(dSc.as<double>(), n2, st);
             prof.end("gw_w_sqrt_v", st);
         }

@@ -58,9 +58,9 @@ struct VBlock {
 };
 // Dense real V(q) (n2 = n, Gamma) or its real embedding [[Re, -Im], [Im, Re]] (n2 = 2n) from the
 // upper blocks, Hermitian fill; V must be zeroed first.  Every dense entry is written by at most
-// one (block, element), so the blocks run in parallel.
+// one (block, element), so the blocks run in parallel.  drop_imag embeds Re V only.
 void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, int n, int n2, int embed,
-                      hipStream_t s);
+                      int drop_imag, hipStream_t s);
 void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s);
 void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s);
 

@@ -258,12 +258,13 @@ void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStr
 }
 
 __global__ void v_scatter_kernel(const double2 *__restrict__ blk, const VBlock *__restrict__ desc, double *V,
-                                 long long n, long long n2, int embed) {
+                                 long long n, long long n2, int embed, int drop_imag) {
     const VBlock b = desc[blockIdx.x];
     const int tot = b.na * b.nb;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int mu = e / b.nb, nu = e % b.nb;
-        const double2 v = blk[b.off + e];
+        double2 v = blk[b.off + e];
+        if (drop_imag) v.y = 0.0;
         const long long r = b.r0 + mu, c = b.c0 + nu;
         V[r * n2 + c] = v.x;
         if (!b.diag) V[c * n2 + r] = v.x;
@@ -281,10 +282,10 @@ __global__ void v_scatter_kernel(const double2 *__restrict__ blk, const VBlock *
 }
 
 void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, int n, int n2, int embed,
-                      hipStream_t s) {
+                      int drop_imag, hipStream_t s) {
     if (nblk == 0) return;
     hipLaunchKernelGGL(v_scatter_kernel, dim3(nblk), dim3(256), 0, s, reinterpret_cast<const double2 *>(blk), d, V,
-                       (long long)n, (long long)n2, embed);
+                       (long long)n, (long long)n2, embed, drop_imag);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -747,7 +747,7 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         prof.begin("ec_v_fill", stream);
         HIP_CHECK(hipMemsetAsync(dV.get(), 0, nn * 8, stream));
         launch_v_scatter(d_vblk.as<double>(), d_vdesc.as<VBlock>() + vdesc_q[iq], vdesc_q[iq + 1] - vdesc_q[iq],
-                         dV.as<double>(), n, n2, embed ? 1 : 0, stream);
+                         dV.as<double>(), n, n2, embed ? 1 : 0, 0, stream);
         prof.end("ec_v_fill", stream, 0, (double)nn * 8);
         prof.begin("ec_chol_v", stream);
         cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream);
