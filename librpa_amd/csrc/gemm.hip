@@ -281,7 +281,7 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
                 if (split) col += (long long)(gn / P.ncb) * P.cbs;
                 T *dst = C + crow + col;
                 const T v0 = alpha * acc[a][b][r];
-                if (VAR == 2 && atomic) {
+                if ((VAR == 2 || VAR == 3) && atomic) {
                     atomicAdd(dst, v0);
                 } else {
                     T v = v0;
@@ -392,20 +392,21 @@ static void launch_mode(const GemmBatch &b, const void *A, const void *B, void *
         launch_one<MODE, 0, true, false, 1>(b, A, B, C, alpha, beta, s);
         return;
     }
+    if (b.tag == 3 || b.tag == 4) {
+        // same code as the plain kernel, instantiated separately so that traces can tell it apart
+        if (b.trans_b || (b.trans_a && b.tag == 4) || !b.coltab.empty() || (b.atomic && b.tag == 4))
+            LRA_THROW("tagged GEMM: plain B, no column table (and plain A, plain epilogue for tag 4)");
+        if (b.tag == 3 && b.trans_a) launch_cfg<MODE, true, false, 3>(b, A, B, C, alpha, beta, s);
+        else if (b.tag == 3) launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
+        else launch_cfg<MODE, false, false, 4>(b, A, B, C, alpha, beta, s);
+        return;
+    }
     if (b.atomic || !b.coltab.empty()) {
         // column-table / atomic batches: plain A and B, 64x64 or 128x128 tiles
         if (b.trans_a || b.trans_b || (b.cfg != T128x128 && b.cfg != T64x64))
             LRA_THROW("column-table GEMM: plain A and B, 64x64 or 128x128 tiles");
         if (b.cfg == T64x64) launch_one<MODE, 0, false, false, 2>(b, A, B, C, alpha, beta, s);
         else launch_one<MODE, 1, false, false, 2>(b, A, B, C, alpha, beta, s);
-        return;
-    }
-    if (b.tag == 3 || b.tag == 4) {
-        // same code as the plain kernel, instantiated separately so that traces can tell it apart
-        if (b.trans_b || (b.trans_a && b.tag == 4)) LRA_THROW("tagged GEMM: plain B (and plain A for tag 4)");
-        if (b.tag == 3 && b.trans_a) launch_cfg<MODE, true, false, 3>(b, A, B, C, alpha, beta, s);
-        else if (b.tag == 3) launch_cfg<MODE, false, false, 3>(b, A, B, C, alpha, beta, s);
-        else launch_cfg<MODE, false, false, 4>(b, A, B, C, alpha, beta, s);
         return;
     }
     if (b.trans_a && b.trans_b) launch_cfg<MODE, true, true>(b, A, B, C, alpha, beta, s);
@@ -443,7 +444,7 @@ std::vector<long long> tune_key(Prec prec, const GemmBatch &b) {
 
 int GemmBatch::tune(Prec prec, const void *A, const void *B, void *C, double alpha, double beta, hipStream_t s,
                     const std::function<void()> &before) {
-    if (probs.empty() || atomic) return cfg;
+    if (probs.empty() || (atomic && tag != 3)) return cfg;
     const auto key = tune_key(prec, *this);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

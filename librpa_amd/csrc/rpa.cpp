@@ -53,6 +53,9 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     ec_prec = pe;
     planned_ntau = ntau;
     ts = (prec == Prec::F64) ? 8 : 4;
+    // K parts of each contraction problem (LIBRPA_AMD_SPLITK, default 4; 1 = one plain write)
+    int nsk = 4;
+    if (const char *e = std::getenv("LIBRPA_AMD_SPLITK")) nsk = std::max(1, std::atoi(e));
     const auto &mf = ds.mf;
     natom = (int)ds.nb_wfc.size();
     if (natom == 0 || ds.nb_aux.size() != (size_t)natom) LRA_THROW("atomic basis sizes not set");
@@ -322,14 +325,21 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
                         if (c.ta_ptr.empty()) c.ta_ptr.push_back(0);
                         c.ta_ptr.push_back((int)c.ta_outs.size());
                     }
-        // final: chi0s(o) = Cfin_I (na_I x ksum_I) * T_o (ksum_I x na_J)
+        // final: chi0s(o) = Cfin_I (na_I x ksum_I) * T_o (ksum_I x na_J), K split in nsk parts that
+        // accumulate atomically: each output block has few, long tiles (K ~ 25k), and shorter ones
+        // fill the last wave of a launch better
         for (int o = c.o_begin; o < c.o_end; ++o) {
             const auto &O = outs[o];
             if (ksum[O.I] == 0) continue;
             const int cls = (na[O.I] > 64 ? 2 : 0) + (na[O.J] > 64 ? 1 : 0);
-            c.fin[cls].add(make_prob(na[O.I], na[O.J], (int)ksum[O.I], cfin_off[O.I], na[O.I], O.toff,
-                                     na[O.J], O.eoff, na[O.J]));
-            c.fin_owner[cls].push_back(o);
+            const int kt = (int)ksum[O.I];
+            const int step = std::max(32, (kt / nsk + 31) / 32 * 32);
+            for (int k0 = 0; k0 < kt; k0 += step) {
+                const int kl = std::min(step, kt - k0);
+                c.fin[cls].add(make_prob(na[O.I], na[O.J], kl, cfin_off[O.I] + (long long)k0 * na[O.I], na[O.I],
+                                         O.toff + (long long)k0 * na[O.J], na[O.J], O.eoff, na[O.J]));
+                c.fin_owner[cls].push_back(o);
+            }
         }
         c.wb.finalize(big_tile(prec), stream);
         c.xb.trans_a = true;
@@ -339,6 +349,7 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
         c.fin_flops = 0;
         for (auto &f : c.fin) {
             f.tag = 3;
+            f.atomic = nsk > 1;
             f.trans_a = true;  // Cfin_I is stored k-major, so A tiles load along mu
             f.finalize(T64x64, stream);
             c.fin_flops += f.flops;
@@ -618,11 +629,11 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
             }
             for (auto &c : chunks) {
                 if (tune_now && c.ta.cfg_by_prec[(int)cur] < 0) {
-                    // TA overwrites T; the contraction trials overwrite chi0s with beta = 0 before
-                    // the X term below resets it for this tau
+                    // TA overwrites T; the contraction trials accumulate into chi0s before the X
+                    // term below resets it (beta = 0) for this tau
                     c.ta.tune(cur, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                     gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                    for (auto &f : c.fin) f.tune(cur, Cfin_p, d_T.get(), chi_base, 1.0, 0.0, stream);
+                    for (auto &f : c.fin) f.tune(cur, Cfin_p, d_T.get(), chi_base, 1.0, 1.0, stream);
                 }
                 prof.begin(rn("chi0_X"), stream);
                 gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
