@@ -118,8 +118,9 @@ def test_bound_screening_is_exact_on_kept_blocks(lib):
 @pytest.mark.gpu
 def test_bound_screening_with_mixed_precision(lib):
     # the mixed mode's FP32 tau points take their G-block norms from the FP32 G panel; screened
-    # and unscreened mixed builds must agree on every kept block, and no skipped block may
-    # exceed the threshold (up to FP32 rounding of the norms)
+    # and unscreened mixed builds must agree on every kept block (to FP32 rounding at the FP32
+    # tau points: the contraction's K parts accumulate atomically, in no fixed order), and no
+    # skipped block may exceed the threshold (up to FP32 rounding of the norms)
     _say("bound screening cell2, mixed precision")
     s = synth.cell2()
     h0, o, ec0 = _run(s, keep=True, prec=L.MIXED)
@@ -127,6 +128,8 @@ def test_bound_screening_with_mixed_precision(lib):
     _, t = h64.get_screen_samples()
     thr = float(np.exp(np.median(t)))
     h1, _, ec1 = _run(s, keep=True, mode=L.SCREEN_BOUND, thr=thr, prec=L.MIXED)
+    grid = h0.get_tfgrids(o)
+    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= 1e3)
     st = h1.get_screen_stats()
     _say(f"  threshold {thr:.2e}: skipped {st['skipped']:.0f} of {st['blocks']:.0f} blocks, dEc {ec1 - ec0:.3e}")
     assert st["skipped"] > 0
@@ -140,7 +143,7 @@ def test_bound_screening_with_mixed_precision(lib):
                     nz += 1
                     assert np.linalg.norm(a) < thr * (1 + 1e-5)
                 else:
-                    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+                    assert np.abs(a - b).max() <= (1e-6 if it in low else 1e-13) * np.abs(a).max()
     assert nz == st["skipped"]
     assert abs(ec1 - ec0) / s.natom < 1e-5
 
