@@ -53,7 +53,9 @@ void librpa_amd_profile_reset(void);
 
 /* Counters of the last chi0 build: [0] algorithmic chi0 flops (canonical count, one spin,
  * all tau), [1] executed GEMM flops, [2] chi0 wall seconds, [3] Ec wall seconds,
- * [4] output blocks (I<=J, R), [5] Cs blocks after folding, [6] n_abf, [7] n_basis.
+ * [4] output blocks (I<=J, R), [5] Cs blocks after folding, [6] n_abf, [7] n_basis,
+ * [8] the mixed mode's cut: tau points whose cosine-transform column stays at or below it run in
+ * FP32 (1e3, or LIBRPA_AMD_MIXED_AMAX), [9] tau points of the last build that ran in FP32.
  * Returns the number of entries written. */
 int librpa_amd_get_stats(LibrpaHandler *h, double *out, int n);
 

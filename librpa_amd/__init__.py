@@ -26,6 +26,8 @@ PREC = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "mixed": MIXED}
 
 # LibrpaTimeFreqGrid / LibrpaParallelRouting values (librpa_enums.h)
 MINIMAX = 3
+RTAU = 0
+ATOMPAIR = 1
 LIBRI = 2
 AUTO = -51
 UNSET = -101
@@ -447,10 +449,10 @@ class Handler:
         return X, t
 
     def get_stats(self):
-        out = np.zeros(8)
-        self._call("librpa_amd_get_stats", out.ctypes.data_as(_dp), 8)
+        out = np.zeros(10)
+        self._call("librpa_amd_get_stats", out.ctypes.data_as(_dp), 10)
         keys = ["flops_chi0_alg", "flops_chi0_exec", "t_chi0", "t_ec", "n_out_blocks", "n_cs_blocks", "n_abf",
-                "n_basis"]
+                "n_basis", "mixed_amax", "n_fp32_tau"]
         return dict(zip(keys, out.tolist()))
 
 

@@ -528,6 +528,8 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             d.stats[5] = (double)eng->n_cs_blocks;
             d.stats[6] = eng->NABF;
             d.stats[7] = eng->NAO;
+            d.stats[8] = eng->mixed_amax;
+            d.stats[9] = eng->n_lowp_tau;
             d.rpa = eng;
             return r.ec_re;
         },
@@ -826,7 +828,7 @@ int librpa_amd_get_stats(LibrpaHandler *h, double *out, int n) {
     return guarded(
         [&]() -> int {
             auto &d = ds_of(h);
-            const int m = std::min(n, 8);
+            const int m = std::min(n, 10);
             for (int i = 0; i < m; ++i) out[i] = d.stats[i];
             return m;
         },

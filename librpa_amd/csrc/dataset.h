@@ -76,7 +76,7 @@ struct Dataset {
     TFGrid grid;
     std::shared_ptr<RpaEngine> rpa;
     std::shared_ptr<GwEngine> gw;
-    double stats[8] = {0};
+    double stats[10] = {0};
 };
 
 // minimax / other grids (tfgrids.cpp)

@@ -524,6 +524,10 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     const double budget = std::min(160e9, 0.7 * (double)free_b) - 8.0 * 10 * nn;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
     DevBuf dV0(nn * 8), dL(nn * 8), dD(nn * 8 * WCH), dX(nn * 8 * WCH), dA(NN * 8 * WCH), dacc(sizeof(double) * (WCH + 8));
+    // pivot signs of v = L S L^T (v_c = v): W needs v^{1/2}, which a v with a negative pivot lacks
+    // (power_hemat, matrix_m.h:916-970, would take the square root of a negative eigenvalue)
+    DevBuf dSg((size_t)n2 * 8);
+    std::vector<double> hsg;
     int *dflag = reinterpret_cast<int *>(dacc.as<double>() + WCH + 4);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), st));
     CholWork cw;
@@ -567,7 +571,13 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
             dense_coulomb(vst, iq, n, n2, embed, gamma, dV0.as<double>(), st);
             HIP_CHECK(hipMemcpyAsync(dL.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
             prof.begin("gw_w_chol_v", st);
-            cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st);
+            cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st,
+                            dSg.as<double>());
+            hsg.resize(n2);
+            HIP_CHECK(hipMemcpyAsync(hsg.data(), dSg.get(), (size_t)n2 * 8, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            for (double v : hsg)
+                if (v < 0) LRA_THROW("W needs a positive-definite Coulomb matrix (negative pivot of v, no v^1/2)");
             prof.end("gw_w_chol_v", st, (double)n2 * n2 * n2 / 3.0);
         } else {
             // dL <- v^{1/2}; dV0 <- v_c, dSc <- v_c^{1/2} (complex v_c kept, as the reference does)

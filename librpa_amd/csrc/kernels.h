@@ -45,7 +45,8 @@ void launch_chi0_assemble(const TS *chi0s, long long E_tot, int ntau, const Pair
                           int nw, TD *dense, long long dense_stride, int n, bool embed, hipStream_t s);
 
 // Dense helpers --------------------------------------------------------------------
-template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s);
+// A = I, or diag(d) when d (n device doubles) is given
+template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s, const double *d = nullptr);
 template <class T> void launch_zero_upper(T *A, int n, int lda, hipStream_t s);
 template <class TS, class TD> void launch_convert(const TS *src, TD *dst, long long n, hipStream_t s);
 void launch_copy_lower_f2d(const float *src, double *dst, int n, int lda, hipStream_t s);
@@ -61,24 +62,26 @@ struct VBlock {
 // one (block, element), so the blocks run in parallel.  drop_imag embeds Re V only.
 void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, int n, int n2, int embed,
                       int drop_imag, hipStream_t s);
-void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s);
+// *acc += sum_i A_ii, or sum_i signs_i A_ii when signs (n device doubles) is given
+void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s, const double *signs = nullptr);
 void launch_copy_block(const double *src, int lds, double *dst, int ldd, int rows, int cols, hipStream_t s);
 
 // Blocked Cholesky with signed pivots (lower, row-major, FP64): A = L S L^T in place,
 // S = diag(+-1); equal to Cholesky for positive-definite A.  logdet (device scalar)
 // += ln |det A|; *flag set to 1 on a zero / NaN pivot.  When keep_L is false only the
-// log-determinant is produced.
+// log-determinant is produced.  signs (n device doubles, optional) receives S.
 struct CholWork {
     DevBuf linv, panel;
 };
 void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w,
-                     hipStream_t s);
+                     hipStream_t s, double *signs = nullptr);
 // The same for nbatch matrices at A + b * strideA, all factorised in the same launches;
 // matrix b adds its ln|det| to logdet[b * ld_stride].  With ncols in (0, n) only the
 // first ncols columns are eliminated: the lower triangle of the trailing block then
 // holds its Schur complement A22 - A21 A11^{-1} A21^T and logdet gets ln|det A11|.
+// Matrix b writes its pivot signs to signs[b * sstride + j] when signs is given.
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
                              long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s,
-                             int ncols = -1);
+                             int ncols = -1, double *signs = nullptr, long long sstride = 0);
 
 }  // namespace lra

@@ -204,18 +204,19 @@ template void launch_chi0_assemble<double, float>(const double *, long long, int
 // dense helpers
 // ------------------------------------------------------------------------------------
 template <class T>
-__global__ void set_identity_kernel(T *A, int n, int lda) {
+__global__ void set_identity_kernel(T *A, int n, int lda, const double *__restrict__ diag) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n * n) return;
     const int i = (int)(idx / n), j = (int)(idx % n);
-    A[(long long)i * lda + j] = (i == j) ? T(1) : T(0);
+    A[(long long)i * lda + j] = (i == j) ? (diag ? static_cast<T>(diag[i]) : T(1)) : T(0);
 }
-template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s) {
-    hipLaunchKernelGGL((set_identity_kernel<T>), dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, n, lda);
+template <class T> void launch_set_identity(T *A, int n, int lda, hipStream_t s, const double *diag) {
+    hipLaunchKernelGGL((set_identity_kernel<T>), dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, n, lda,
+                       diag);
     HIP_CHECK(hipGetLastError());
 }
-template void launch_set_identity<double>(double *, int, int, hipStream_t);
-template void launch_set_identity<float>(float *, int, int, hipStream_t);
+template void launch_set_identity<double>(double *, int, int, hipStream_t, const double *);
+template void launch_set_identity<float>(float *, int, int, hipStream_t, const double *);
 
 template <class T>
 __global__ void zero_upper_kernel(T *A, int n, int lda) {
@@ -289,10 +290,11 @@ void launch_v_scatter(const double *blk, const VBlock *d, int nblk, double *V, i
     HIP_CHECK(hipGetLastError());
 }
 
-__global__ void trace_kernel(const double *__restrict__ A, int n, int lda, double *acc) {
+__global__ void trace_kernel(const double *__restrict__ A, int n, int lda, double *acc,
+                             const double *__restrict__ signs) {
     __shared__ double red[256];
     double v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += A[(long long)i * lda + i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += (signs ? signs[i] : 1.0) * A[(long long)i * lda + i];
     red[threadIdx.x] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -301,8 +303,8 @@ __global__ void trace_kernel(const double *__restrict__ A, int n, int lda, doubl
     }
     if (threadIdx.x == 0) *acc += red[0];
 }
-void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s) {
-    hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(256), 0, s, A, n, lda, acc);
+void launch_trace_acc(const double *A, int n, int lda, double *acc, hipStream_t s, const double *signs) {
+    hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(256), 0, s, A, n, lda, acc, signs);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -333,7 +335,7 @@ constexpr int CHOL_NB = 64;
 // columns tx + 16 b of the block.
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long strideA, int lda, int j0, int nb,
                                                          double *work, double *logdet0, long long ld_stride,
-                                                         int *flag) {
+                                                         int *flag, double *signs, long long sstride) {
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
     __shared__ double X[CHOL_NB][CHOL_NB + 1];
     __shared__ double S[CHOL_NB];
@@ -357,7 +359,9 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long s
             L[j][j] = sqrt(fabs(d));
         }
         __syncthreads();
-        if (t > j && t < nb) L[t][j] /= L[j][j];
+        // L_tj = a_tj / (S_j sqrt|d_j|), so that A = L S L^T also for a negative pivot; the
+        // update a_ik -= S_j L_ij L_kj below then equals a_ik - a_ij a_kj / d_j
+        if (t > j && t < nb) L[t][j] *= S[j] / L[j][j];
         __syncthreads();
         const double sj = S[j];
         for (int a = 0; a < 4; ++a) {
@@ -393,6 +397,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long s
             Linv[i * CHOL_NB + k] = (i < nb && k < nb) ? X[i][k] : 0.0;
         }
     if (t < CHOL_NB) sgn[t] = t < nb ? S[t] : 1.0;
+    if (signs != nullptr && t < nb) signs[sstride * blockIdx.x + j0 + t] = S[t];
     if (t == 0) {
         double sum = 0;
         for (int j = 0; j < nb; ++j) sum += 2.0 * log(L[j][j]);
@@ -432,7 +437,8 @@ static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan
 // matrix.  Q and Q S of the outer panel's rows are kept in two wide buffers (pitch CHOL_PW, indexed
 // by global row) per matrix.
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
-                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols) {
+                             long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols,
+                             double *signs, long long sstride) {
     const int nf = (ncols > 0 && ncols < n) ? ncols : n;
     const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
     const long long pstride = 2LL * n * CHOL_PW;  // Q then Q S, each n x CHOL_PW
@@ -495,7 +501,7 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
         const int p0 = (j0 / CHOL_PW) * CHOL_PW;
         hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(256), 0, s, A, strideA, lda, j0, nb, work, logdet,
-                           ld_stride, flag);
+                           ld_stride, flag, signs, sstride);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
         if (m > 0) {
@@ -516,8 +522,9 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
         for (int bb = 0; bb < nbatch; ++bb) launch_zero_upper<double>(A + strideA * bb, n, lda, s);
 }
 
-void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s) {
-    cholesky_logdet_batched(A, 0, 1, n, lda, logdet, 0, flag, keep_L, w, s);
+void cholesky_logdet(double *A, int n, int lda, double *logdet, int *flag, bool keep_L, CholWork &w, hipStream_t s,
+                     double *signs) {
+    cholesky_logdet_batched(A, 0, 1, n, lda, logdet, 0, flag, keep_L, w, s, -1, signs, 0);
 }
 
 }  // namespace lra

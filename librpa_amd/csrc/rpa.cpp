@@ -19,7 +19,8 @@
 // and the R -> q sum (chi_libri_ft_ct, chi0.cpp:507-596) are fused into the
 // assembly of dense Hermitian chi0(q, omega) matrices, and Ec
 // (compute_RPA_correlation_blacs_2d{,_gamma_only}, src/core/epsilon.cpp:43-558)
-// uses ln det(1 - V chi0) = ln det(1 - L^T chi0 L) with V = L L^T.
+// uses ln |det(1 - V chi0)| = ln |det(S - L^T chi0 L)| with V = L S L^T (S = diag(+-1), the
+// pivot signs; S = I for a positive-definite V).
 #include "rpa.h"
 
 #include <algorithm>
@@ -552,6 +553,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         for (auto &f : c.fin) f.use_skip = false;
     }
     bool tuned_chi0[2] = {false, false};  // [plan precision, FP32 of the mixed mode]
+    n_lowp_tau = 0;
     for (int it = 0; it < ntau; ++it) {
         const double tau = g.tau[it];
         // mixed precision: FP32 contractions for the tau points whose column of the cosine t->f
@@ -563,6 +565,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
             if (amax <= mixed_amax) cur = Prec::F32;
         }
         const bool lowp = cur != prec;
+        n_lowp_tau += lowp ? 1 : 0;
         // low-precision tau points time under their own region names
         auto rn = [lowp](const char *r) { return lowp ? std::string(r) + "_f32" : std::string(r); };
         const double bsc = lowp ? 4.0 / ts : 1.0;  // algorithmic bytes scale with the element size
@@ -700,6 +703,10 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     const double budget = std::min(96e9, 0.6 * (double)free_b) - (double)nn * 12;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
     DevBuf dV(nn * 8), dL32, dD(nn * es * WCH), dX(nn * es * WCH), dM(nn * es * WCH), dM64, dacc;
+    // pivot signs S of V = L S L^T: a Coulomb matrix that is not positive definite keeps
+    // det(1 - V chi0) = det(S) det(S - L^T chi0 L) and Tr(V chi0) = n - sum_i S_i M_ii with
+    // M = S - L^T chi0 L, so Ec is exact for any V (as the LU of epsilon.cpp:600-693 is)
+    DevBuf dS((size_t)n2 * 8);
     if (!f64) { dL32.alloc(nn * 4); dM64.alloc(nn * 8 * WCH); }
     dacc.alloc(sizeof(double) * 2 * nq * nw + 64);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
@@ -727,7 +734,7 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         HIP_CHECK(hipStreamSynchronize(stream));
         v_ibz = ibz;
     }
-    // (chunk size, order, precision) -> (X = D L, M = I - L^T X), kept with their tuned tiles
+    // (chunk size, order, precision) -> (X = D L, M = S - L^T X), kept with their tuned tiles
     auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
         const auto key = std::make_tuple(cnt, n2, (int)ec_prec);
         auto it = ec_batches.find(key);
@@ -761,7 +768,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
                          dV.as<double>(), n, n2, embed ? 1 : 0, 0, stream);
         prof.end("ec_v_fill", stream, 0, (double)nn * 8);
         prof.begin("ec_chol_v", stream);
-        cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream);
+        cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream,
+                        dS.as<double>());
         prof.end("ec_chol_v", stream, (double)n2 * n2 * n2 / 3.0);
         const void *Lp = dV.get();
         if (!f64) {
@@ -817,8 +825,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
                 gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
                 bb.second.tune(ec_prec, Lp, dX.get(), dM.get(), -1.0, 1.0, stream, [&]() {
                     for (int w = 0; w < cnt; ++w) {
-                        if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream);
-                        else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream);
+                        if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
+                        else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
                     }
                 });
             }
@@ -826,8 +834,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
             prof.end("ec_gemm_x", stream, bb.first.flops);
             for (int w = 0; w < cnt; ++w) {
-                if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream);
-                else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream);
+                if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
+                else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
             }
             prof.begin("ec_gemm_m", stream);
             gemm_run(ec_prec, bb.second, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
@@ -841,7 +849,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             }
             double *acc0 = dacc.as<double>() + 2 * ((size_t)iq * nw + w0);
             prof.begin("ec_logdet", stream);
-            for (int w = 0; w < cnt; ++w) launch_trace_acc(M64 + (size_t)w * nn, n2, n2, acc0 + 2 * w, stream);
+            for (int w = 0; w < cnt; ++w)
+                launch_trace_acc(M64 + (size_t)w * nn, n2, n2, acc0 + 2 * w, stream, dS.as<double>());
             cholesky_logdet_batched(M64, (long long)nn, cnt, n2, n2, acc0 + 1, 2, dflag, false, cw, stream);
             prof.end("ec_logdet", stream, (double)cnt * n2 * n2 * n2 / 3.0);
         }

@@ -300,6 +300,104 @@ def spin_polarised(s: System, seed: int = 0, shift: float = 0.03) -> System:
                                efermi=float(0.5 * (homo + lumo)), nspin=2)
 
 
+def _minus_k(kfrac, period):
+    """Index of -k (mod 1) for each k of a Gamma-centred mesh."""
+    idx = {tuple(int(round(v * p)) % p for v, p in zip(k, period)): i for i, k in enumerate(kfrac)}
+    return [idx[tuple((-int(round(v * p))) % p for v, p in zip(k, period))] for k in kfrac]
+
+
+def time_reversal_symmetric(s: System, seed: int = 0) -> System:
+    """The same system with time-reversal-symmetric k-space inputs: for every pair (k, -k)
+    eps(-k) = eps(k), U(-k) = conj U(k) and v(-k) = conj v(k); at the k points equal to their own
+    -k (components 0 or 1/2) U and v are real.  G(R) and the BvK-supercell Coulomb matrix are then
+    real, so the Gamma-only path of the supercell (real parts only) loses nothing."""
+    if s.nspin != 1:
+        raise ValueError("time_reversal_symmetric: one spin channel")
+    rng = np.random.default_rng(seed + 91)
+    mk = _minus_k(s.kfrac, s.period)
+    eps, occ, wfc = s.eps.copy(), s.occ.copy(), s.wfc.copy()
+    vq = None if s.vq is None else s.vq.copy()
+    nb = s.wfc.shape[3]
+    for ik, jk in enumerate(mk):
+        if jk < ik:
+            continue
+        if jk == ik:
+            q, r = np.linalg.qr(rng.standard_normal((nb, nb)))
+            wfc[0, ik] = (q * np.sign(np.diag(r))[None, :]).T
+            if vq is not None:
+                vq[ik] = vq[ik].real.astype(np.complex128)
+        else:
+            eps[0, jk], occ[0, jk] = eps[0, ik], occ[0, ik]
+            wfc[0, jk] = wfc[0, ik].conj()
+            if vq is not None:
+                vq[jk] = vq[ik].conj()
+    return dataclasses.replace(s, name=s.name + "_tr", eps=eps, occ=occ, wfc=wfc, vq=vq)
+
+
+def bvk_supercell(s: System) -> System:
+    """The Gamma-only BvK supercell (period x cell) of a k-mesh system, by the Bloch convention
+    psi_nk = sum_R e^{i 2 pi k.R} sum_i U_k(n, i) phi_i(r - R) / sqrt(N) for the eigenvectors and
+    chi(q) = sum_R e^{i 2 pi q.R} chi(0, R) for auxiliary-basis matrices:
+
+    * supercell atom (r, I) sits at tau_I + R_r . A, R_r on the BvK grid (``r_grid`` order);
+    * eigenvector of state (k, n) on basis function (r, I, i): e^{i 2 pi k.R_r} U_k(n, (I, i)) / sqrt(N);
+      states sorted by energy;
+    * Cs block (I, K, R1) becomes ((r, I), (r', K), L) with R_r + R1 = R_r' + period * L;
+    * Coulomb matrix v((r, I, mu), (r', J, nu)) = (1 / N) sum_q e^{-i 2 pi q.(R_r' - R_r)} v(q)_{(I,mu),(J,nu)}.
+
+    The RPA correlation energy of the supercell is then N times the per-cell Ec of the k mesh."""
+    if s.nspin != 1:
+        raise ValueError("bvk_supercell: one spin channel")
+    per = np.array(s.period)
+    N = int(per.prod())
+    cells = [(x, y, z) for x in range(-(per[0] // 2), (per[0] - 1) // 2 + 1)
+             for y in range(-(per[1] // 2), (per[1] - 1) // 2 + 1)
+             for z in range(-(per[2] // 2), (per[2] - 1) // 2 + 1)]
+    cidx = {c: i for i, c in enumerate(cells)}
+    na = s.natom
+    lat = s.latvec
+    pos = np.concatenate([s.positions + np.array(c, dtype=float) @ lat for c in cells])
+    types = np.concatenate([s.types for _ in cells])
+    nao = np.concatenate([s.nao for _ in cells])
+    nabf = np.concatenate([s.nabf for _ in cells])
+    ph_k = np.exp(2j * np.pi * (s.kfrac @ np.array(cells, dtype=float).T))   # (nk, N)
+    nbasis = s.nbasis
+    # states (k, n) on basis (r, b)
+    Z = (ph_k[:, None, :, None] * s.wfc[0][:, :, None, :]).reshape(s.nk * s.nbands, N * nbasis) / np.sqrt(N)
+    e = s.eps[0].reshape(-1)
+    o = s.occ[0].reshape(-1)
+    order = np.lexsort((np.arange(e.size), e))
+    eps = e[order][None, None, :]
+    occ = o[order][None, None, :]
+    wfc = Z[order][None, None]
+
+    def fold(v):
+        out = []
+        for a in range(3):
+            p = int(per[a])
+            f = ((v[a] % p) + (3 * p) // 2) % p - p // 2
+            out.append((f, (v[a] - f) // p))
+        return tuple(x[0] for x in out), tuple(x[1] for x in out)
+
+    cs = []
+    for (I, K, R1, blk) in s.cs:
+        for r, c in enumerate(cells):
+            f, L = fold([c[a] + R1[a] for a in range(3)])
+            cs.append((r * na + I, cidx[f] * na + K, L, blk.copy()))
+    vq = None
+    if s.vq is not None:
+        n = s.nabf_tot
+        ph_q = np.exp(-2j * np.pi * (s.kfrac @ np.array(cells, dtype=float).T))  # (nq, N)
+        # V[(r, mu), (r', nu)] = (1/N) sum_q e^{-i 2 pi q.(R_r' - R_r)} v(q)
+        V = np.einsum('qa,qb,qmn->ambn', ph_q.conj(), ph_q, s.vq).reshape(N * n, N * n) / N
+        if np.abs(V.imag).max() > 1e-10 * np.abs(V.real).max():
+            raise ValueError("bvk_supercell: v(q) is not time-reversal symmetric (complex supercell v)")
+        vq = V.real.astype(np.complex128)[None]
+    return System(name=s.name + "_bvk", latvec=lat * per[:, None], positions=pos, types=types, nao=nao,
+                  nabf=nabf, period=(1, 1, 1), kfrac=np.zeros((1, 3)), eps=eps, occ=occ, efermi=s.efermi,
+                  wfc=wfc, cs=cs, vq=vq, nfreq=s.nfreq)
+
+
 WORKLOADS = {
     "cell2": cell2,
     "cell16": cell16,

@@ -114,7 +114,7 @@ def test_cell16_small_chunks_match_oracle(system, golden, fp64, monkeypatch, pre
         assert abs(ec - d["ec_ha"]) / system.natom < 1e-5
     grid = h.get_tfgrids(o)
     for it in (0, 12, 17):
-        low = prec == L.MIXED and np.abs(grid["cos_t2f"][:, it]).max() <= 1e3
+        low = prec == L.MIXED and np.abs(grid["cos_t2f"][:, it]).max() <= h.get_stats()["mixed_amax"]
         ref = fp64["blocks"][it]
         got = _blocks(h, system, it)
         err = max(np.abs(got[k] - ref[k]).max() for k in ref) / max(np.abs(b).max() for b in ref.values())
@@ -150,7 +150,9 @@ def test_cell16_mixed_precision(system, golden, fp64):
     h, o, ec, _ = _run(system, L.MIXED)
     grid = h.get_tfgrids(o)
     amax = np.abs(grid["cos_t2f"]).max(axis=0)
-    low = [t for t in range(system.nfreq) if amax[t] <= 1e3]
+    st = h.get_stats()
+    low = [t for t in range(system.nfreq) if amax[t] <= st["mixed_amax"]]
+    assert len(low) == st["n_fp32_tau"]
     _say(f"  FP32 tau points {low}; Ec {ec!r} oracle {d['ec_ha']!r}; dEc/atom {(ec - d['ec_ha']) / system.natom:.3e} Ha")
     assert 0 < len(low) < system.nfreq
     for it in sorted(set(CHECK_TAU) | set(REDUCED_TAU)):

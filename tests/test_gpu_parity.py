@@ -132,7 +132,9 @@ def test_mixed_precision_against_fp64(case):
     s = CASES[case]()
     h64, _, ec64, _, grid = _run_lib(s)
     h, _, ec, contrib, _ = _run_lib(s, L.MIXED)
-    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= 1e3)
+    st = h.get_stats()
+    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= st["mixed_amax"])
+    assert len(low) == st["n_fp32_tau"]
     assert 0 < len(low) < s.nfreq
     e64 = e32 = 0.0
     for it in range(s.nfreq):

@@ -129,7 +129,9 @@ def test_bound_screening_with_mixed_precision(lib):
     thr = float(np.exp(np.median(t)))
     h1, _, ec1 = _run(s, keep=True, mode=L.SCREEN_BOUND, thr=thr, prec=L.MIXED)
     grid = h0.get_tfgrids(o)
-    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= 1e3)
+    st0 = h0.get_stats()
+    low = set(t for t in range(s.nfreq) if np.abs(grid["cos_t2f"][:, t]).max() <= st0["mixed_amax"])
+    assert len(low) == st0["n_fp32_tau"]
     st = h1.get_screen_stats()
     _say(f"  threshold {thr:.2e}: skipped {st['skipped']:.0f} of {st['blocks']:.0f} blocks, dEc {ec1 - ec0:.3e}")
     assert st["skipped"] > 0
@@ -150,11 +152,17 @@ def test_bound_screening_with_mixed_precision(lib):
 
 @pytest.mark.gpu
 def test_learned_screening_gain_cell16(lib):
-    _say("learned screening cell16")
-    s = synth.cell16()
-    h0, _, ec0 = _run(s, keep=True)
-    X, t = h0.get_screen_samples()
+    # trained on the block norms of one 16-atom cell (seed 1), applied to a held-out one (seed 6:
+    # other jitter, eigenvalues, eigenvectors, Cs and v); dEc against the held-out cell's own
+    # unscreened build
+    _say("learned screening cell16, held-out system")
+    train = synth.cell16()
+    ht, _, _ = _run(train, keep=True)
+    X, t = ht.get_screen_samples()
     model, loss = L.screen_train(X, t, iters=1500, lr=3e-3, seed=0)
+    del ht
+    s = synth.cell16(seed=6)
+    h0, _, ec0 = _run(s)
     thr = 1e-8
     hb, _, ec_b = _run(s, mode=L.SCREEN_BOUND, thr=thr)
     h2, _, ec_l = _run(s, mode=L.SCREEN_LEARNED, thr=thr, model=model)
@@ -163,6 +171,6 @@ def test_learned_screening_gain_cell16(lib):
     _say(f"  train mse {loss:.2e} on {len(t)} samples; skipped blocks: bound {fb:.3f}, learned {fl:.3f}; "
          f"flops skipped: bound {sb['flops_skipped'] / sb['flops']:.3f}, learned {sl['flops_skipped'] / sl['flops']:.3f}; "
          f"dEc/atom bound {(ec_b - ec0) / s.natom:.2e}, learned {(ec_l - ec0) / s.natom:.2e}")
-    assert fl >= fb
+    assert fl > 0 and fl >= fb
     assert abs(ec_b - ec0) / s.natom < 1e-5
     assert abs(ec_l - ec0) / s.natom < 1e-5
