@@ -100,6 +100,18 @@ double librpa_amd_screen_train(const float *X, const float *t, int n, int iters,
  * b2[64], w3[64], b3.  Return 0 on success. */
 int librpa_amd_mlp_predict(const float *params, const float *X, int n, float *y);
 int librpa_amd_mlp_loss_grad(const float *params, const float *X, const float *t, int n, float *grad, double *loss);
+/* No counterpart in LibRPA.  librpa_amd_mlp_loss_grad for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight retraining loops): the same kernels on
+ * device-resident arrays d_params [4801], d_X [n][8], d_t [n] (in) and d_grad [4801], d_loss
+ * [1 double] (out), enqueued on stream (a hipStream_t; NULL = the null stream) without
+ * synchronising.  The per-sample activations and chunk partial sums live in a static device
+ * array of the library (35 MB, held by each GPU it runs on until the process ends), so
+ * n <= LIBRPA_AMD_MLP_NOALLOC_MAX; a larger n is an error.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race.  Returns 0 when
+ * the work is enqueued, nonzero on error. */
+#define LIBRPA_AMD_MLP_NOALLOC_MAX 32768
+int librpa_amd_mlp_loss_grad_noalloc(const float *d_params, const float *d_X, const float *d_t, int n, float *d_grad,
+                                     double *d_loss, void *stream);
 
 /* One dense GEMM C = alpha op(A) op(B) + beta C on host arrays through the
  * device engine (prec as above; tile_cfg 0..8 selects the workgroup tile).

@@ -165,6 +165,8 @@ _SIGS = {
     "librpa_amd_screen_train": (C.c_double, [_fp, _fp, C.c_int, C.c_int, C.c_double, C.c_int, _fp]),
     "librpa_amd_mlp_predict": (C.c_int, [_fp, _fp, C.c_int, _fp]),
     "librpa_amd_mlp_loss_grad": (C.c_int, [_fp, _fp, _fp, C.c_int, _fp, _dp]),
+    "librpa_amd_mlp_loss_grad_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]),
 }
 
 # block-norm screener (librpa_amd.h): MLP sizes
@@ -172,6 +174,7 @@ SCREEN_NFEAT = 8
 SCREEN_NPARAM = 4801
 SCREEN_MODEL = SCREEN_NPARAM + 2 * SCREEN_NFEAT + 2
 SCREEN_OFF, SCREEN_BOUND, SCREEN_LEARNED = 0, 1, 2
+MLP_NOALLOC_MAX = 32768  # LIBRPA_AMD_MLP_NOALLOC_MAX
 
 
 def load(path: Optional[str] = None):
@@ -236,6 +239,15 @@ def mlp_loss_grad(params, X, t):
     load().librpa_amd_mlp_loss_grad(pp, xp, tp, len(t), g.ctypes.data_as(_fp), loss.ctypes.data_as(_dp))
     _check()
     return float(loss[0]), g
+
+
+def mlp_loss_grad_noalloc(d_params, d_X, d_t, n, d_grad, d_loss, stream=None):
+    """librpa_amd_mlp_loss_grad_noalloc on device addresses (ints); enqueued on ``stream`` (a
+    hipStream_t address, None for the null stream), not synchronised.  Raises on error."""
+    rc = load().librpa_amd_mlp_loss_grad_noalloc(d_params, d_X, d_t, int(n), d_grad, d_loss, stream)
+    _check()
+    if rc != 0:
+        raise LibrpaError(f"librpa_amd_mlp_loss_grad_noalloc returned {rc}")
 
 
 def init_global():

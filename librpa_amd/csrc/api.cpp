@@ -146,6 +146,7 @@ void librpa_init_global(LibrpaSwitch, const char *, LibrpaSwitch) {
         HIP_CHECK(hipGetDeviceCount(&n));
         if (n < 1) LRA_THROW("no GPU visible");
         HIP_CHECK(hipSetDevice(0));
+        (void)mlp_noalloc_setup();  // address of the allocation-free entry's work array on device 0
     });
 }
 
@@ -667,6 +668,16 @@ int librpa_amd_mlp_loss_grad(const float *params, const float *X, const float *t
             mlp_loss_grad(dp.as<float>(), dx.as<float>(), dt.as<float>(), n, dg.as<float>(), dl.as<double>(), w, 0);
             HIP_CHECK(hipMemcpy(grad, dg.get(), sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
             HIP_CHECK(hipMemcpy(loss, dl.get(), sizeof(double), hipMemcpyDeviceToHost));
+            return 0;
+        },
+        1);
+}
+
+int librpa_amd_mlp_loss_grad_noalloc(const float *d_params, const float *d_X, const float *d_t, int n, float *d_grad,
+                                     double *d_loss, void *stream) {
+    return guarded(
+        [&]() -> int {
+            mlp_loss_grad_noalloc(d_params, d_X, d_t, n, d_grad, d_loss, static_cast<hipStream_t>(stream));
             return 0;
         },
         1);

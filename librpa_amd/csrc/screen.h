@@ -38,6 +38,15 @@ struct MlpWork {
 // grad and loss are device pointers (loss: one double)
 void mlp_loss_grad(const float *params, const float *X, const float *t, int n, float *grad, double *loss,
                    MlpWork &w, hipStream_t s);
+// The same on device arrays without allocating: the work memory is a static device array of the
+// library (kMlpNoallocScratchBytes per GPU), for n <= kMlpNoallocMax; larger n throws.  Calls that
+// overlap on one GPU share that array and race.
+constexpr int kMlpNoallocMax = 32768;
+constexpr size_t kMlpNoallocScratchBytes = 35176704;  // activations + dy, r2, 64 chunk partials
+void mlp_loss_grad_noalloc(const float *params, const float *X, const float *t, int n, float *grad, double *loss,
+                           hipStream_t s);
+// Device address of the static work array on the current device (resolved once per device).
+char *mlp_noalloc_setup();
 // one Adam step on params with gradient grad (step >= 1)
 void launch_adam(float *params, const float *grad, float *m, float *v, int n, float lr, int step, hipStream_t s);
 

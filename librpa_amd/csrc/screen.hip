@@ -7,6 +7,10 @@
 // over sample chunks, a third sums the chunks; Adam updates the parameters in place.
 #include "screen.h"
 
+#include <map>
+#include <mutex>
+#include <string>
+
 namespace lra {
 
 namespace {
@@ -224,27 +228,77 @@ __global__ __launch_bounds__(1024) void mean_kernel(const double *__restrict__ v
     if (threadIdx.x == 0) out[0] = tot / (double)n;
 }
 
+namespace {
+// The loss and gradient launches on caller-provided work memory: act [n][ACT] followed by dy [n],
+// r2 [n], part [GCHUNKS][MLP_NPARAM].  Every entry the later launches read is written by an earlier
+// one in the same call, so the work memory needs no clearing.
+void loss_grad_launches(const float *params, const float *X, const float *t, int n, float *grad, double *loss,
+                        float *act, double *r2, float *part, hipStream_t s) {
+    float *dy = act + (size_t)n * ACT;
+    hipLaunchKernelGGL(mlp_fwdbwd_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, params, X, t, n, act, dy, r2);
+    HIP_CHECK(hipGetLastError());
+    const int chunk = ceil_div(n, GCHUNKS);
+    const int nch = ceil_div(n, chunk);
+    hipLaunchKernelGGL(mlp_grad_rows_kernel, dim3(NROW, nch), dim3(256), 0, s, X, act, dy, n, chunk, part);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3(ceil_div(MLP_NPARAM, 256)), dim3(256), 0, s, part, nch, grad);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, s, r2, n, loss);
+    HIP_CHECK(hipGetLastError());
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kActBytes = align256(sizeof(float) * (size_t)kMlpNoallocMax * (ACT + 1));
+constexpr size_t kR2Bytes = align256(sizeof(double) * (size_t)kMlpNoallocMax);
+constexpr size_t kPartBytes = align256(sizeof(float) * (size_t)GCHUNKS * MLP_NPARAM);
+constexpr size_t kScratchBytes = kActBytes + kR2Bytes + kPartBytes;
+static_assert(kScratchBytes == kMlpNoallocScratchBytes, "scratch size");
+}  // namespace
+
+// Work memory of the allocation-free loss and gradient (about 35 MB): each GPU the library runs
+// on holds its own copy from the code object's load until the process ends.  Host code reaches it
+// only through the per-device address from hipGetSymbolAddress below, never through this name.
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_dev;  // device ordinal -> device address of scratch
+}  // namespace
+
+char *mlp_noalloc_setup() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_dev.find(dev);
+    if (it != g_scratch_dev.end()) return it->second;
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || p == nullptr)
+        LRA_THROW(std::string("hipGetSymbolAddress of the MLP scratch array failed: ") + hipGetErrorString(rc));
+    g_scratch_dev[dev] = static_cast<char *>(p);
+    return static_cast<char *>(p);
+}
+
+void mlp_loss_grad_noalloc(const float *params, const float *X, const float *t, int n, float *grad, double *loss,
+                           hipStream_t s) {
+    if (n <= 0) LRA_THROW("mlp: no samples");
+    if (n > kMlpNoallocMax)
+        LRA_THROW("allocation-free MLP loss/gradient: " + std::to_string(n) + " samples exceed the maximum of " +
+                  std::to_string(kMlpNoallocMax));
+    if (!params || !X || !t || !grad || !loss) LRA_THROW("allocation-free MLP loss/gradient: null device pointer");
+    char *base = mlp_noalloc_setup();
+    loss_grad_launches(params, X, t, n, grad, loss, reinterpret_cast<float *>(base),
+                       reinterpret_cast<double *>(base + kActBytes), reinterpret_cast<float *>(base + kActBytes + kR2Bytes),
+                       s);
+}
+
 void mlp_loss_grad(const float *params, const float *X, const float *t, int n, float *grad, double *loss,
                    MlpWork &w, hipStream_t s) {
     if (n <= 0) LRA_THROW("mlp: no samples");
     w.acts.reserve(sizeof(float) * ((size_t)n * ACT + n));
     w.r2.reserve(sizeof(double) * n);
     w.part.reserve(sizeof(float) * (size_t)GCHUNKS * MLP_NPARAM);
-    float *act = w.acts.as<float>();
-    float *dy = act + (size_t)n * ACT;
-    hipLaunchKernelGGL(mlp_fwdbwd_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, params, X, t, n, act, dy,
-                       w.r2.as<double>());
-    HIP_CHECK(hipGetLastError());
-    const int chunk = ceil_div(n, GCHUNKS);
-    const int nch = ceil_div(n, chunk);
-    hipLaunchKernelGGL(mlp_grad_rows_kernel, dim3(NROW, nch), dim3(256), 0, s, X, act, dy, n, chunk,
-                       w.part.as<float>());
-    HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(mlp_grad_sum_kernel, dim3(ceil_div(MLP_NPARAM, 256)), dim3(256), 0, s, w.part.as<float>(), nch,
-                       grad);
-    HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, s, w.r2.as<double>(), n, loss);
-    HIP_CHECK(hipGetLastError());
+    loss_grad_launches(params, X, t, n, grad, loss, w.acts.as<float>(), w.r2.as<double>(), w.part.as<float>(), s);
 }
 
 __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
