@@ -128,8 +128,30 @@ def dominant_kernel(prof, prec_name, workload):
                       **common)
 
 
+def cpu_baseline_measured(L, reps=3):
+    """The FP64 oracle run in full on the 2-atom cell (2x2x2 k, nfreq 16), median of reps runs,
+    with the FP64 GPU path timed on the same workload in the same run."""
+    from librpa_amd import synth
+    from oracle import rpa
+    s = synth.cell2()
+    times, ec_cpu = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ec_cpu = rpa.rpa_from_system(s)["ec"]
+        times.append(time.perf_counter() - t0)
+    h = L.Handler()
+    L.load_system(h, s)
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    ec_gpu, tg, _, _ = run_steps(L, h, o, s.nk, L.FP64, 3, 1)
+    t_cpu, t_gpu = float(np.median(times)), float(np.median(tg))
+    return {"workload": "cell2 (2 atoms, 2x2x2 k, nfreq 16)", "runs": reps,
+            "cpu_s_per_freq": round(t_cpu / s.nfreq, 4), "gpu_fp64_s_per_freq": round(t_gpu / s.nfreq, 6),
+            "ratio": round(t_cpu / t_gpu, 1), "ec_cpu_ha": ec_cpu, "ec_gpu_ha": ec_gpu}
+
+
 def cpu_baseline(sys_, grid, nfreq):
-    """FP64 oracle timed on a bounded sample of the workload and scaled to s/freq."""
+    """FP64 oracle timed on a bounded sample of the workload and scaled to s/freq (an
+    extrapolation: a full oracle run of this workload takes hours)."""
     from oracle import rpa
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cs = rpa.fold_cs(sys_)
@@ -170,7 +192,7 @@ def cpu_baseline(sys_, grid, nfreq):
     rpa.ln_det_ref(np.eye(m) - Pi)
     t_ec1 = (time.perf_counter() - t0) * (n / m) ** 3
     total = chi0_per_tau * len(grid["tau"]) + t_ec1 * nfreq * sys_.nk
-    return dict(value=round(total / nfreq, 3), unit="s/freq", cores=threads, kind="port",
+    return dict(value=round(total / nfreq, 3), unit="s/freq", cores=threads, kind="port", extrapolated=True,
                 sample=(f"numpy FP64 oracle: G(+-tau) and W(J={J}) plus {nI} output blocks at tau {it}, "
                         f"scaled to {n_out} blocks x {len(grid['tau'])} tau; Ec: one omega at n={m} scaled "
                         f"by (n_abf/{m})^3; total / nfreq"))
@@ -272,28 +294,31 @@ def main():
     o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
     grid = h.get_tfgrids(o)
 
-    # plain FP64 GPU path
-    ec64, t64, prof64, st64 = run_steps(L, h, o, s.nk, L.FP64, 1, 1)
+    # plain FP64 GPU path, timed over as many steps as the chosen path (at least 3)
+    n64 = max(3, args.steps)
+    ec64, t64, prof64, st64 = run_steps(L, h, o, s.nk, L.FP64, n64, max(1, args.warmup))
     s64 = float(np.median(t64))
-    _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step  chi0 {st64['t_chi0']:.3f}s  Ec {st64['t_ec']:.3f}s")
+    _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step (median of {n64})  chi0 {st64['t_chi0']:.3f}s  "
+         f"Ec {st64['t_ec']:.3f}s")
 
     cand = ["bf16x3", "fp32", "mixed"] if args.prec == "auto" else [args.prec]
     chosen = None
     for pn in cand:
         if pn == "fp64":
-            ec, times, prof, st = ec64, t64, prof64, st64
-            if args.steps > 1 or args.warmup > 0:
-                ec, times, prof, st = run_steps(L, h, o, s.nk, L.FP64, args.steps, args.warmup)
-        else:
-            ec, times, prof, st = run_steps(L, h, o, s.nk, L.PREC[pn], args.steps, args.warmup)
+            chosen = (pn, ec64, t64, prof64, st64, 0.0)
+            break
+        # one evaluation (tuning included) decides whether the mode meets the Ec bar
+        ec, _, _, _ = run_steps(L, h, o, s.nk, L.PREC[pn], 1, 0)
         dec = abs(ec - ec64) / s.natom
-        _log(f"{pn}: Ec {ec:.10f}  dEc/atom {dec:.3e}  {np.median(times):.3f} s/step")
+        _log(f"{pn}: Ec {ec:.10f}  dEc/atom {dec:.3e}")
         if dec <= EC_TOL_PER_ATOM or args.prec != "auto":
+            ec, times, prof, st = run_steps(L, h, o, s.nk, L.PREC[pn], args.steps, args.warmup)
+            dec = abs(ec - ec64) / s.natom
+            _log(f"{pn}: Ec {ec:.10f}  dEc/atom {dec:.3e}  {np.median(times):.3f} s/step")
             chosen = (pn, ec, times, prof, st, dec)
             break
     if chosen is None:
-        ec, times, prof, st = run_steps(L, h, o, s.nk, L.FP64, args.steps, args.warmup)
-        chosen = ("fp64", ec, times, prof, st, 0.0)
+        chosen = ("fp64", ec64, t64, prof64, st64, 0.0)
     pn, ec, times, prof, st, dec = chosen
     step = float(np.median(times))
     total_flops = st["flops_chi0_alg"]
@@ -317,6 +342,7 @@ def main():
         "ec_fp64_ha": ec64,
         "dEc_per_atom_ha": dec,
         "fp64_s_per_freq": round(s64 / s.nfreq, 5),
+        "fp64_steps": n64,
         "speedup_vs_fp64_gpu": round(s64 / step, 3),
         "chi0_tflops_effective": round(total_flops / st["t_chi0"] / 1e12, 3) if st["t_chi0"] > 0 else None,
         "t_chi0_s": round(st["t_chi0"], 4),
@@ -335,6 +361,9 @@ def main():
         t0 = time.time()
         res["cpu_baseline"] = cpu_baseline(s, grid, s.nfreq)
         _log(f"cpu baseline sampled in {time.time() - t0:.1f}s")
+        t0 = time.time()
+        res["cpu_baseline"]["measured"] = cpu_baseline_measured(L)
+        _log(f"cpu baseline measured on cell2 in {time.time() - t0:.1f}s: {res['cpu_baseline']['measured']}")
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res), flush=True)

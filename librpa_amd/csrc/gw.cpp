@@ -520,6 +520,8 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     const size_t nn = (size_t)n2 * n2, NN = (size_t)N * N;
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    // chi0(R, omega) once for all q when it fits in a quarter of the free memory
+    if (!R0.chiw_ok && R0.form_chiw(g, 0.25 * (double)free_b, st)) HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const double per_w = 8.0 * (NN + 2.0 * nn);
     const double budget = std::min(160e9, 0.7 * (double)free_b) - 8.0 * 10 * nn;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
@@ -603,30 +605,7 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
         for (int w0 = 0; w0 < nw; w0 += WCH) {
             const int cnt = std::min(WCH, nw - w0);
             prof.begin("gw_w_chi0q", st);
-            for (int a0 = 0; a0 < cnt; a0 += 8) {
-                const int cw_n = std::min(8, cnt - a0);
-                std::vector<double> gp((size_t)cw_n * nR * R0.ntau_done * 2, 0.0);
-                for (int w = 0; w < cw_n; ++w)
-                    for (int rr = 0; rr < nR; ++rr)
-                        for (int t = 0; t < R0.ntau_done; ++t) {
-                            const double gam = g.cos_t2f[(size_t)(w0 + a0 + w) * g.n + t] * 2.0 / nspin;
-                            gp[(((size_t)w * nR + rr) * R0.ntau_done + t) * 2] = gam * csr[rr];
-                            gp[(((size_t)w * nR + rr) * R0.ntau_done + t) * 2 + 1] = gam * snr[rr];
-                        }
-                DevBuf dgp = upload(gp, st);
-                double *Dd = dD.as<double>() + (size_t)a0 * nn;
-                if (R0.prec == Prec::F64)
-                    launch_chi0_assemble<double, double>(R0.d_chi0s.as<double>(), R0.E_tot, R0.ntau_done,
-                                                         R0.d_pairs.as<PairDesc>(), (int)R0.pairs.size(),
-                                                         R0.n_pair_elems, R0.d_pair_eoff.as<long long>(), nR,
-                                                         dgp.as<double>(), cw_n, Dd, (long long)nn, n, embed, st);
-                else
-                    launch_chi0_assemble<float, double>(R0.d_chi0s.as<float>(), R0.E_tot, R0.ntau_done,
-                                                        R0.d_pairs.as<PairDesc>(), (int)R0.pairs.size(),
-                                                        R0.n_pair_elems, R0.d_pair_eoff.as<long long>(), nR,
-                                                        dgp.as<double>(), cw_n, Dd, (long long)nn, n, embed, st);
-                HIP_CHECK(hipStreamSynchronize(st));
-            }
+            R0.assemble_dense(g, kf, w0, cnt, dD.get(), true, (long long)nn, embed, st);
             prof.end("gw_w_chi0q", st);
             auto &bb = get_batches(cnt);
             prof.begin("gw_w_gemm", st);

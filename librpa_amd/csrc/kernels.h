@@ -39,10 +39,24 @@ struct PairDesc {
     int rowI, colJ;    // ABF offsets
     long long e0;      // first element in the flat pair-element index
 };
+constexpr int kChi0AssembleMaxW = 8;  // frequencies per launch (nw <= this)
 template <class TS, class TD>
 void launch_chi0_assemble(const TS *chi0s, long long E_tot, int ntau, const PairDesc *pairs, int npairs,
                           long long n_pair_elems, const long long *eoff, int nR, const double *gam_ph,
                           int nw, TD *dense, long long dense_stride, int n, bool embed, hipStream_t s);
+
+// The same in two passes, when a chi0(R, omega) store fits: chiw[w][i] = sum_t coef[w][t]
+// chi0s[t][i] once for all q (coef = (2 / nspin) gamma_c, nw x ntau, row-major), then per q and
+// frequency chunk the R -> q phase sum ph[r] = (cos, sin)(2 pi q.R_r) into dense matrices.
+constexpr int kTauOmegaMax = 34;  // the largest minimax grid
+void launch_tau_to_omega(const double *chi0s, long long E_tot, int ntau, const double *coef, int nw, double *chiw,
+                         hipStream_t s);
+void launch_tau_to_omega(const float *chi0s, long long E_tot, int ntau, const double *coef, int nw, double *chiw,
+                         hipStream_t s);
+template <class TD>
+void launch_chi0_rq(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs, long long n_pair_elems,
+                    const long long *eoff, int nR, const double *ph, int nw, TD *dense, long long dense_stride, int n,
+                    bool embed, hipStream_t s);
 
 // Dense helpers --------------------------------------------------------------------
 // A = I, or diag(d) when d (n device doubles) is given

@@ -116,9 +116,9 @@ template void launch_gather_cols<float>(const float *, long long, float *, const
 // ------------------------------------------------------------------------------------
 // chi0 (R, tau) -> dense chi0 (q, omega)
 // ------------------------------------------------------------------------------------
-constexpr int kMaxW = 8;
-
-template <class TS, class TD>
+// NW accumulators per thread (a multiple of 8 covering the chunk): one pass over chi0(R, tau) per
+// chunk of up to kChi0AssembleMaxW frequencies
+template <class TS, class TD, int NW>
 __global__ void chi0_assemble_kernel(const TS *__restrict__ chi0s, long long E_tot, int ntau,
                                      const PairDesc *__restrict__ pairs, int npairs, long long nelem,
                                      const long long *__restrict__ eoff, int nR, const double *__restrict__ gp,
@@ -133,16 +133,16 @@ __global__ void chi0_assemble_kernel(const TS *__restrict__ chi0s, long long E_t
     const PairDesc P = pairs[lo];
     const long long e = idx - P.e0;
     const int mu = (int)(e / P.naJ), nu = (int)(e % P.naJ);
-    double re[kMaxW], im[kMaxW];
+    double re[NW], im[NW];
 #pragma unroll
-    for (int w = 0; w < kMaxW; ++w) { re[w] = 0; im[w] = 0; }
+    for (int w = 0; w < NW; ++w) { re[w] = 0; im[w] = 0; }
     for (int r = 0; r < nR; ++r) {
         const long long off = eoff[(long long)lo * nR + r];
         if (off < 0) continue;
         for (int t = 0; t < ntau; ++t) {
             const double v = static_cast<double>(chi0s[(long long)t * E_tot + off + e]);
 #pragma unroll
-            for (int w = 0; w < kMaxW; ++w) {
+            for (int w = 0; w < NW; ++w) {
                 if (w < nw) {
                     const double *g = gp + (((long long)w * nR + r) * ntau + t) * 2;
                     re[w] += g[0] * v;
@@ -154,7 +154,7 @@ __global__ void chi0_assemble_kernel(const TS *__restrict__ chi0s, long long E_t
     const long long row = P.rowI + mu, col = P.colJ + nu;
     const bool mirror = P.I != P.J;
 #pragma unroll
-    for (int w = 0; w < kMaxW; ++w) {
+    for (int w = 0; w < NW; ++w) {
         if (w >= nw) break;
         TD *D = dense + w * dstride;
         if (!embed) {
@@ -181,10 +181,10 @@ template <class TS, class TD>
 void launch_chi0_assemble(const TS *chi0s, long long E_tot, int ntau, const PairDesc *pairs, int npairs,
                           long long n_pair_elems, const long long *eoff, int nR, const double *gam_ph, int nw,
                           TD *dense, long long dense_stride, int n, bool embed, hipStream_t s) {
-    if (nw > kMaxW) LRA_THROW("omega chunk too large");
-    hipLaunchKernelGGL((chi0_assemble_kernel<TS, TD>), dim3(ceil_div(n_pair_elems, 256)), dim3(256), 0, s, chi0s,
-                       E_tot, ntau, pairs, npairs, n_pair_elems, eoff, nR, gam_ph, nw, dense, dense_stride, n,
-                       (int)embed);
+    if (nw < 1 || nw > kChi0AssembleMaxW) LRA_THROW("omega chunk too large");
+    hipLaunchKernelGGL((chi0_assemble_kernel<TS, TD, kChi0AssembleMaxW>), dim3(ceil_div(n_pair_elems, 256)), dim3(256),
+                       0, s, chi0s, E_tot, ntau, pairs, npairs, n_pair_elems, eoff, nR, gam_ph, nw, dense, dense_stride,
+                       n, (int)embed);
     HIP_CHECK(hipGetLastError());
 }
 template void launch_chi0_assemble<double, double>(const double *, long long, int, const PairDesc *, int, long long,
@@ -199,6 +199,114 @@ template void launch_chi0_assemble<float, double>(const float *, long long, int,
 template void launch_chi0_assemble<double, float>(const double *, long long, int, const PairDesc *, int, long long,
                                                   const long long *, int, const double *, int, float *, long long,
                                                   int, bool, hipStream_t);
+
+// chiw[w][i] = sum_t coef[w][t] chi0s[t][i] for every element i of the chi0(R, tau) store: the
+// cosine tau -> omega transform, done once for all q.  Coefficients in LDS; each thread owns one
+// element and re-reads its ntau values once per group of 8 frequencies.
+template <class TS>
+__global__ __launch_bounds__(256) void tau_to_omega_kernel(const TS *__restrict__ chi0s, long long E_tot, int ntau,
+                                                           const double *__restrict__ coef, int nw,
+                                                           double *__restrict__ chiw) {
+    __shared__ double c[kTauOmegaMax * kTauOmegaMax];
+    for (int k = threadIdx.x; k < nw * ntau; k += blockDim.x) c[k] = coef[k];
+    __syncthreads();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < E_tot;
+         i += (long long)gridDim.x * blockDim.x) {
+        for (int w0 = 0; w0 < nw; w0 += 8) {
+            double acc[8];
+#pragma unroll
+            for (int w = 0; w < 8; ++w) acc[w] = 0.0;
+            for (int t = 0; t < ntau; ++t) {
+                const double v = static_cast<double>(chi0s[(long long)t * E_tot + i]);
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    if (w0 + w < nw) acc[w] += c[(w0 + w) * ntau + t] * v;
+            }
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+                if (w0 + w < nw) chiw[(long long)(w0 + w) * E_tot + i] = acc[w];
+        }
+    }
+}
+
+void launch_tau_to_omega(const double *chi0s, long long E_tot, int ntau, const double *coef, int nw, double *chiw,
+                         hipStream_t s) {
+    if (ntau > kTauOmegaMax || nw > kTauOmegaMax) LRA_THROW("tau -> omega: grid too large");
+    const int bx = (int)std::min<long long>(ceil_div(E_tot, 256), 8192);
+    hipLaunchKernelGGL(tau_to_omega_kernel<double>, dim3(bx), dim3(256), 0, s, chi0s, E_tot, ntau, coef, nw, chiw);
+    HIP_CHECK(hipGetLastError());
+}
+void launch_tau_to_omega(const float *chi0s, long long E_tot, int ntau, const double *coef, int nw, double *chiw,
+                         hipStream_t s) {
+    if (ntau > kTauOmegaMax || nw > kTauOmegaMax) LRA_THROW("tau -> omega: grid too large");
+    const int bx = (int)std::min<long long>(ceil_div(E_tot, 256), 8192);
+    hipLaunchKernelGGL(tau_to_omega_kernel<float>, dim3(bx), dim3(256), 0, s, chi0s, E_tot, ntau, coef, nw, chiw);
+    HIP_CHECK(hipGetLastError());
+}
+
+// dense chi0(q, omega) of nw frequencies from chi0(R, omega): per pair element and frequency,
+// sum_r (cos_r + i sin_r) chiw[w][eoff[p][r] + e], written like chi0_assemble (Hermitian fill,
+// real or embedded).  Reads chiw once per (element, R, frequency).
+template <class TD>
+__global__ __launch_bounds__(256) void chi0_rq_kernel(const double *__restrict__ chiw, long long E_tot,
+                                                      const PairDesc *__restrict__ pairs, int npairs, long long nelem,
+                                                      const long long *__restrict__ eoff, int nR,
+                                                      const double *__restrict__ ph, int nw, TD *__restrict__ dense,
+                                                      long long dstride, int n, int embed) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nelem) return;
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].e0 <= idx) lo = mid; else hi = mid - 1;
+    }
+    const PairDesc P = pairs[lo];
+    const long long e = idx - P.e0;
+    const int mu = (int)(e / P.naJ), nu = (int)(e % P.naJ);
+    const long long row = P.rowI + mu, col = P.colJ + nu;
+    const bool mirror = P.I != P.J;
+    for (int w = 0; w < nw; ++w) {
+        double re = 0, im = 0;
+        const double *cw = chiw + (long long)w * E_tot + e;
+        for (int r = 0; r < nR; ++r) {
+            const long long off = eoff[(long long)lo * nR + r];
+            if (off < 0) continue;
+            const double v = cw[off];
+            re += ph[2 * r] * v;
+            im += ph[2 * r + 1] * v;
+        }
+        TD *D = dense + w * dstride;
+        if (!embed) {
+            D[row * n + col] = static_cast<TD>(re);
+            if (mirror) D[col * n + row] = static_cast<TD>(re);
+        } else {
+            const long long ld = 2ll * n;
+            D[row * ld + col] = static_cast<TD>(re);
+            D[(n + row) * ld + n + col] = static_cast<TD>(re);
+            D[(n + row) * ld + col] = static_cast<TD>(im);
+            D[row * ld + n + col] = static_cast<TD>(-im);
+            if (mirror) {
+                D[col * ld + row] = static_cast<TD>(re);
+                D[(n + col) * ld + n + row] = static_cast<TD>(re);
+                D[(n + col) * ld + row] = static_cast<TD>(-im);
+                D[col * ld + n + row] = static_cast<TD>(im);
+            }
+        }
+    }
+}
+
+template <class TD>
+void launch_chi0_rq(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs, long long n_pair_elems,
+                    const long long *eoff, int nR, const double *ph, int nw, TD *dense, long long dense_stride, int n,
+                    bool embed, hipStream_t s) {
+    hipLaunchKernelGGL((chi0_rq_kernel<TD>), dim3(ceil_div(n_pair_elems, 256)), dim3(256), 0, s, chiw, E_tot, pairs,
+                       npairs, n_pair_elems, eoff, nR, ph, nw, dense, dense_stride, n, (int)embed);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_chi0_rq<double>(const double *, long long, const PairDesc *, int, long long, const long long *,
+                                     int, const double *, int, double *, long long, int, bool, hipStream_t);
+template void launch_chi0_rq<float>(const double *, long long, const PairDesc *, int, long long, const long long *, int,
+                                    const double *, int, float *, long long, int, bool, hipStream_t);
 
 // ------------------------------------------------------------------------------------
 // dense helpers
@@ -327,81 +435,83 @@ void launch_copy_block(const double *src, int lds, double *dst, int ldd, int row
 // ------------------------------------------------------------------------------------
 constexpr int CHOL_NB = 64;
 
-// Factor the nb x nb diagonal block at (j0, j0) of matrix blockIdx.x in LDS as
-// L S L^T with S = diag(+-1) (a Cholesky factorisation that tolerates negative
-// pivots), write L11 back, write Linv = L11^{-1} (row-major, CHOL_NB pitch) and the
-// pivot signs, and accumulate sum log|d_j| = ln |det| of the block.  A zero or NaN
-// pivot sets *flag.  Thread (ty, tx) of a 16 x 16 layout owns rows ty + 16 a and
-// columns tx + 16 b of the block.
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A0, long long strideA, int lda, int j0, int nb,
-                                                         double *work, double *logdet0, long long ld_stride,
-                                                         int *flag, double *signs, long long sstride) {
+// A double of another lane (lane is wave-uniform).
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffu), lane));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Factor the nb x nb diagonal block at (j0, j0) of matrix blockIdx.x as L S L^T with
+// S = diag(+-1) (a Cholesky factorisation that tolerates negative pivots), write L11 back, write
+// Linv = L11^{-1} (row-major, CHOL_NB pitch) and the pivot signs, and accumulate
+// sum log|d_j| = ln |det| of the block.  A zero or NaN pivot sets *flag.
+//
+// One wave per matrix; lane t keeps row t of the unit-padded 64 x 64 block in registers (the
+// loops are unrolled so every index is static).  At step j every lane takes the pivot from lane j
+// with readlane and forms s_j and sqrt|d_j| itself, lane t > j scales its entry, and each lane
+// updates its row with the scaled column read lane by lane.  The inverse is then formed with lane c
+// holding column c, again from readlane of the rows.  (Rows in LDS with lane-local loops took
+// three times as long: every step then waits on LDS round trips.)
+__global__ __launch_bounds__(64) void potrf_diag_kernel(double *A0, long long strideA, int lda, int j0, int nb,
+                                                        double *work, double *logdet0, long long ld_stride,
+                                                        int *flag, double *signs, long long sstride) {
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
-    __shared__ double X[CHOL_NB][CHOL_NB + 1];
-    __shared__ double S[CHOL_NB];
-    __shared__ int bad;
-    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int t = threadIdx.x;
     double *A = A0 + strideA * blockIdx.x;
     double *Linv = work + (long long)blockIdx.x * (CHOL_NB * CHOL_NB + CHOL_NB);
     double *sgn = Linv + CHOL_NB * CHOL_NB;
-    if (t == 0) bad = 0;
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-            const int i = ty + 16 * a, k = tx + 16 * b;
-            L[i][k] = (i < nb && k < nb) ? A[(long long)(j0 + i) * lda + j0 + k] : (i == k ? 1.0 : 0.0);
-        }
+    // coalesced row loads through LDS into row t of lane t
+    for (int r = 0; r < CHOL_NB; ++r)
+        L[r][t] = (r < nb && t < nb) ? A[(long long)(j0 + r) * lda + j0 + t] : (r == t ? 1.0 : 0.0);
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (t == 0) {
-            double d = L[j][j];
-            if (!(fabs(d) > 0.0)) { bad = 1; d = 1.0; }
-            S[j] = d > 0 ? 1.0 : -1.0;
-            L[j][j] = sqrt(fabs(d));
-        }
-        __syncthreads();
-        // L_tj = a_tj / (S_j sqrt|d_j|), so that A = L S L^T also for a negative pivot; the
-        // update a_ik -= S_j L_ij L_kj below then equals a_ik - a_ij a_kj / d_j
-        if (t > j && t < nb) L[t][j] *= S[j] / L[j][j];
-        __syncthreads();
-        const double sj = S[j];
-        for (int a = 0; a < 4; ++a) {
-            const int i = ty + 16 * a;
-            if (i <= j) continue;
-            const double lij = sj * L[i][j];
-            for (int b = 0; b < 4; ++b) {
-                const int k = tx + 16 * b;
-                if (k > j && k <= i) L[i][k] -= lij * L[k][j];
-            }
-        }
-        __syncthreads();
+    double a[CHOL_NB];
+#pragma unroll
+    for (int k = 0; k < CHOL_NB; ++k) a[k] = L[t][k];
+    bool bad = false;
+    double my_s = 1.0, my_r = 1.0;
+#pragma unroll
+    for (int j = 0; j < CHOL_NB; ++j) {
+        double d = readlane_f64(a[j], j);
+        if (!(fabs(d) > 0.0)) { bad = true; d = 1.0; }
+        const double s = d > 0 ? 1.0 : -1.0;
+        const double r = sqrt(fabs(d));
+        // L_tj = a_tj / (s_j sqrt|d_j|), so that A = L S L^T also for a negative pivot; the update
+        // a_tk -= s_j L_tj L_kj then equals a_tk - a_tj a_kj / d_j
+        const double l = a[j] * (s / r);
+        if (t == j) { a[j] = r; my_s = s; my_r = r; }
+        if (t > j) a[j] = l;
+        const double f = t > j ? s * l : 0.0;
+        // entries k > t (upper triangle) are updated too and never used: no lane mask needed
+#pragma unroll
+        for (int k = j + 1; k < CHOL_NB; ++k) a[k] -= f * readlane_f64(a[j], k);
     }
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-            const int i = ty + 16 * a, k = tx + 16 * b;
-            if (i < nb && k <= i) A[(long long)(j0 + i) * lda + j0 + k] = L[i][k];
-        }
-    // inverse of the unit-padded lower-triangular factor: column c by thread c
-    if (t < CHOL_NB) {
-        const int c = t;
-        for (int r = 0; r < CHOL_NB; ++r) {
-            if (r < c) { X[r][c] = 0; continue; }
-            double x = (r == c) ? 1.0 : 0.0;
-            for (int k = c; k < r; ++k) x -= L[r][k] * X[k][c];
-            X[r][c] = x / L[r][r];
-        }
-    }
+    // L back through LDS as coalesced rows (lower triangle)
     __syncthreads();
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-            const int i = ty + 16 * a, k = tx + 16 * b;
-            Linv[i * CHOL_NB + k] = (i < nb && k < nb) ? X[i][k] : 0.0;
-        }
-    if (t < CHOL_NB) sgn[t] = t < nb ? S[t] : 1.0;
-    if (signs != nullptr && t < nb) signs[sstride * blockIdx.x + j0 + t] = S[t];
+#pragma unroll
+    for (int k = 0; k < CHOL_NB; ++k) L[t][k] = a[k];
+    __syncthreads();
+    for (int r = 0; r < nb; ++r)
+        if (t <= r) A[(long long)(j0 + r) * lda + j0 + t] = L[r][t];
+    // column t of L^{-1}: x_r = (delta_rt - sum_{k < r} L_rk x_k) / L_rr (x_k = 0 for k < t)
+    double x[CHOL_NB];
+    const double dinv = 1.0 / my_r;
+#pragma unroll
+    for (int r = 0; r < CHOL_NB; ++r) {
+        double acc = r == t ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < r; ++k) acc -= readlane_f64(a[k], r) * x[k];
+        x[r] = acc * readlane_f64(dinv, r);
+    }
+#pragma unroll
+    for (int r = 0; r < CHOL_NB; ++r) Linv[r * CHOL_NB + t] = (r < nb && t < nb) ? x[r] : 0.0;
+    sgn[t] = t < nb ? my_s : 1.0;
+    if (signs != nullptr && t < nb) signs[sstride * blockIdx.x + j0 + t] = my_s;
+    double v = t < nb ? 2.0 * log(my_r) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if (t == 0) {
-        double sum = 0;
-        for (int j = 0; j < nb; ++j) sum += 2.0 * log(L[j][j]);
-        logdet0[ld_stride * blockIdx.x] += sum;
+        logdet0[ld_stride * blockIdx.x] += v;
         if (bad) *flag = 1;
     }
 }
@@ -500,8 +610,8 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
     for (size_t b = 0; b < cp.j0s.size(); ++b) {
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
         const int p0 = (j0 / CHOL_PW) * CHOL_PW;
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(256), 0, s, A, strideA, lda, j0, nb, work, logdet,
-                           ld_stride, flag, signs, sstride);
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(CHOL_NB), 0, s, A, strideA, lda, j0, nb, work,
+                           logdet, ld_stride, flag, signs, sstride);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
         if (m > 0) {

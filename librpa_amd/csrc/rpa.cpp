@@ -518,6 +518,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     auto &prof = profiler();
     const double t0 = now_s();
     const int ntau = g.n;
+    chiw_ok = false;  // chi0(R, omega) of an earlier build is stale
     prepare_gf(mf, nbands_g);
     d_Ghat.reserve(ghat_max * ts);
     d_W.reserve((size_t)w_off.back() * ts);
@@ -696,8 +697,15 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     const size_t es = f64 ? 8 : 4;
     const int nw = g.n;
     const int nq = (int)ibz.size();
-    // all omega of a q are factorised together, as far as memory allows
+    // chi0(R, omega) once for all q when it fits in a third of the free memory
     size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    if (!chiw_ok) {
+        prof.begin("chi0_tau_to_omega", stream);
+        form_chiw(g, 0.33 * (double)free_b, stream);
+        prof.end("chi0_tau_to_omega", stream, 0, (double)E_tot * g.n * (ts * ceil_div(g.n, 8) + 8.0));
+    }
+    // all omega of a q are factorised together, as far as memory allows
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const double per_w = (double)nn * (3 * es + (f64 ? 0 : 8));
     const double budget = std::min(96e9, 0.6 * (double)free_b) - (double)nn * 12;
@@ -712,8 +720,6 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
     int *dflag = reinterpret_cast<int *>(dacc.as<char>() + sizeof(double) * 2 * nq * nw);
     CholWork cw;
-    std::vector<double> gp;
-    DevBuf dgp;
     if (v_ibz != ibz) {
         // stage the input's V(q) blocks once per plan (the engine is rebuilt when inputs change)
         std::vector<VBlock> desc;
@@ -779,45 +785,12 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         const double *kf = &ds.kfrac[(size_t)kq * 3];
         for (int w0 = 0; w0 < nw; w0 += WCH) {
             const int cnt = std::min(WCH, nw - w0);
-            // tau -> omega and R -> q of chi0s into dense matrices, at most 8 omega per launch
+            // tau -> omega (formed once above, or per chunk) and R -> q of chi0s into dense matrices
             prof.begin("chi0_transform", stream);
-            for (int a0 = 0; a0 < cnt; a0 += 8) {
-                const int cw_n = std::min(8, cnt - a0);
-                gp.assign((size_t)cw_n * nR * ntau_done * 2, 0.0);
-                for (int w = 0; w < cw_n; ++w)
-                    for (int r = 0; r < nR; ++r) {
-                        const double ang =
-                            2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
-                        for (int t = 0; t < ntau_done; ++t) {
-                            const double gam = g.cos_t2f[(size_t)(w0 + a0 + w) * g.n + t] * 2.0 / nspin;
-                            gp[(((size_t)w * nR + r) * ntau_done + t) * 2] = gam * std::cos(ang);
-                            gp[(((size_t)w * nR + r) * ntau_done + t) * 2 + 1] = gam * std::sin(ang);
-                        }
-                    }
-                dgp = upload(gp, stream);
-                char *Dd = dD.as<char>() + (size_t)a0 * nn * es;
-                if (prec == Prec::F64 && f64)
-                    launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                         (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
-                                                         nR, dgp.as<double>(), cw_n, (double *)Dd, (long long)nn, n,
-                                                         embed, stream);
-                else if (prec == Prec::F64)
-                    launch_chi0_assemble<double, float>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                        (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
-                                                        nR, dgp.as<double>(), cw_n, (float *)Dd, (long long)nn, n,
-                                                        embed, stream);
-                else if (f64)
-                    launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                        (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
-                                                        nR, dgp.as<double>(), cw_n, (double *)Dd, (long long)nn, n,
-                                                        embed, stream);
-                else
-                    launch_chi0_assemble<float, float>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
-                                                       (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(),
-                                                       nR, dgp.as<double>(), cw_n, (float *)Dd, (long long)nn, n,
-                                                       embed, stream);
-            }
-            const double tbytes = (double)ts * E_tot * ntau_done * ceil_div(cnt, 8) + (double)es * cnt * nn;
+            assemble_dense(g, kf, w0, cnt, dD.get(), f64, (long long)nn, embed, stream);
+            const double tbytes = chiw_ok ? 8.0 * E_tot * cnt + (double)es * cnt * nn
+                                          : (double)ts * E_tot * ntau_done * ceil_div(cnt, kChi0AssembleMaxW) +
+                                                (double)es * cnt * nn;
             prof.end("chi0_transform", stream, 0, tbytes);
             auto &bb = get_batches(cnt);
             if (autotune && !bb.first.tuned) {
@@ -874,6 +847,78 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     for (auto &c : res.qcontrib) res.ec_re += c.real();
     t_ec = now_s() - t0;
     return res;
+}
+
+bool RpaEngine::form_chiw(const TFGrid &g, double max_bytes, hipStream_t s) {
+    chiw_ok = false;
+    const double need = 8.0 * (double)E_tot * g.n;
+    if (ntau_done != g.n || g.n > kTauOmegaMax || need > max_bytes) return false;
+    d_chiw.reserve((size_t)need);
+    std::vector<double> coef((size_t)g.n * g.n);
+    for (int w = 0; w < g.n; ++w)
+        for (int t = 0; t < g.n; ++t) coef[(size_t)w * g.n + t] = g.cos_t2f[(size_t)w * g.n + t] * 2.0 / nspin;
+    DevBuf dc = upload(coef, s);
+    if (prec == Prec::F64)
+        launch_tau_to_omega(d_chi0s.as<double>(), E_tot, ntau_done, dc.as<double>(), g.n, d_chiw.as<double>(), s);
+    else
+        launch_tau_to_omega(d_chi0s.as<float>(), E_tot, ntau_done, dc.as<double>(), g.n, d_chiw.as<double>(), s);
+    HIP_CHECK(hipStreamSynchronize(s));  // dc is freed on return
+    chiw_ok = true;
+    return true;
+}
+
+void RpaEngine::assemble_dense(const TFGrid &g, const double *kf, int w0, int cnt, void *dense, bool f64, long long nn,
+                               bool embed, hipStream_t s) {
+    const int n = NABF;
+    std::vector<double> ph((size_t)nR * 2);
+    for (int r = 0; r < nR; ++r) {
+        const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+        ph[2 * r] = std::cos(ang);
+        ph[2 * r + 1] = std::sin(ang);
+    }
+    if (chiw_ok) {
+        DevBuf dph = upload(ph, s);
+        const double *cw = d_chiw.as<double>() + (size_t)w0 * E_tot;
+        if (f64)
+            launch_chi0_rq<double>(cw, E_tot, d_pairs.as<PairDesc>(), (int)pairs.size(), n_pair_elems,
+                                   d_pair_eoff.as<long long>(), nR, dph.as<double>(), cnt, (double *)dense, nn, n, embed, s);
+        else
+            launch_chi0_rq<float>(cw, E_tot, d_pairs.as<PairDesc>(), (int)pairs.size(), n_pair_elems,
+                                  d_pair_eoff.as<long long>(), nR, dph.as<double>(), cnt, (float *)dense, nn, n, embed, s);
+        HIP_CHECK(hipStreamSynchronize(s));  // dph is freed on return
+        return;
+    }
+    const size_t es = f64 ? 8 : 4;
+    for (int a0 = 0; a0 < cnt; a0 += kChi0AssembleMaxW) {
+        const int cw_n = std::min(kChi0AssembleMaxW, cnt - a0);
+        std::vector<double> gp((size_t)cw_n * nR * ntau_done * 2, 0.0);
+        for (int w = 0; w < cw_n; ++w)
+            for (int r = 0; r < nR; ++r)
+                for (int t = 0; t < ntau_done; ++t) {
+                    const double gam = g.cos_t2f[(size_t)(w0 + a0 + w) * g.n + t] * 2.0 / nspin;
+                    gp[(((size_t)w * nR + r) * ntau_done + t) * 2] = gam * ph[2 * r];
+                    gp[(((size_t)w * nR + r) * ntau_done + t) * 2 + 1] = gam * ph[2 * r + 1];
+                }
+        DevBuf dgp = upload(gp, s);
+        char *Dd = static_cast<char *>(dense) + (size_t)a0 * nn * es;
+        if (prec == Prec::F64 && f64)
+            launch_chi0_assemble<double, double>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                 (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                 dgp.as<double>(), cw_n, (double *)Dd, nn, n, embed, s);
+        else if (prec == Prec::F64)
+            launch_chi0_assemble<double, float>(d_chi0s.as<double>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                dgp.as<double>(), cw_n, (float *)Dd, nn, n, embed, s);
+        else if (f64)
+            launch_chi0_assemble<float, double>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                                (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                                dgp.as<double>(), cw_n, (double *)Dd, nn, n, embed, s);
+        else
+            launch_chi0_assemble<float, float>(d_chi0s.as<float>(), E_tot, ntau_done, d_pairs.as<PairDesc>(),
+                                               (int)pairs.size(), n_pair_elems, d_pair_eoff.as<long long>(), nR,
+                                               dgp.as<double>(), cw_n, (float *)Dd, nn, n, embed, s);
+        HIP_CHECK(hipStreamSynchronize(s));  // dgp is freed on return
+    }
 }
 
 void RpaEngine::release_work() {

@@ -130,6 +130,17 @@ struct RpaEngine {
     void build_gf(const MeanField &mf, double tau, int s);
     void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);
     void release_work();  // free the gathered-G, W and T work buffers (rebuilt by build_chi0)
+    // chi0(R, omega) = sum_tau (2 / nspin) gamma_c(omega, tau) chi0(R, tau) of the last build, FP64
+    // [omega][E], formed once per Ec / W evaluation when it fits in max_bytes (returns whether it
+    // was formed); the R -> q sums then read it once per q instead of chi0(R, tau) per chunk of 8
+    // frequencies
+    DevBuf d_chiw;
+    bool chiw_ok = false;
+    bool form_chiw(const TFGrid &g, double max_bytes, hipStream_t s);
+    // dense chi0(q, omega) of the frequencies [w0, w0 + cnt) at fractional q into cnt matrices of nn
+    // elements (FP64 or FP32), real at Gamma or the real embedding
+    void assemble_dense(const TFGrid &g, const double *kf, int w0, int cnt, void *dense, bool f64, long long nn,
+                        bool embed, hipStream_t s);
     long long ghat_max = 0, t_max = 0;
     RpaResult ec(const Dataset &ds, const TFGrid &g);
     // dense chi0(q, omega) block for verification
