@@ -210,34 +210,146 @@ def copy_calib(nbytes):
 
 
 def g0w0_bench(args):
-    """W(q, w) -> W(R, tau) and Sigma_c(R, i w) wall time on one GPU, per precision."""
+    """chi0 -> W -> Sigma_c wall time on one GPU per precision, and the quasi-particle energies of
+    the 8 states around the gap at every k (Pade continuation and the QP fixed point of
+    librpa_get_g0w0_sigc_kgrid with v_xc = v_exx = 0, so e_qp = e_KS + Re Sigma_c(e_qp)); each
+    reduced-precision run is reported as its largest |dE_qp| against the FP64 run, in meV."""
     import librpa_amd as L
     from librpa_amd import synth
     L.init_global()
     wl = args.workload if args.workload != "cubic128" else "cubic64"
     t0 = time.time()
-    s = synth.WORKLOADS[wl]()
+    dev = None
+    if args.gen_device == "auto":
+        try:
+            import torch
+            dev = "cuda" if torch.cuda.is_available() else None
+        except Exception:
+            pass
+    try:
+        s = synth.WORKLOADS[wl](vq_device=dev)
+    except TypeError:
+        s = synth.WORKLOADS[wl]()
     _log(f"workload {wl}: {s.natom} atoms, n_ao {s.nbasis}, n_abf {s.nabf_tot}, nk {s.nk}, nfreq {s.nfreq} "
          f"(generated in {time.time() - t0:.1f}s)")
     h = L.Handler()
     L.load_system(h, s, vq_cut=True)
     o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI, output_level=4)
-    precs = ["fp64"] if args.prec in ("auto", "fp64") else [args.prec]
+    precs = ["fp64", "mixed", "fp32", "bf16x3"] if args.prec == "auto" else \
+        ["fp64"] + ([args.prec] if args.prec != "fp64" else [])
+    nocc = int(round(s.occ[0, 0].sum() / 2.0))
+    lo, hi = nocc - 4, nocc + 4
+    iks = list(range(s.nk))
+    nst = hi - lo
+    zeros = np.zeros(s.nk * nst)
     res = {"metric": "G0W0 chi0 + W + Sigma_c wall s", "unit": "s", "higher_is_better": False, "n_gpus": 1,
            "config": {"workload": wl, "atoms": s.natom, "n_ao": s.nbasis, "n_abf": s.nabf_tot, "nk": s.nk,
-                      "nfreq": s.nfreq}, "data": "synthetic (seeded generator, librpa_amd/synth.py)", "runs": {}}
+                      "nfreq": s.nfreq}, "data": "synthetic (seeded generator, librpa_amd/synth.py)",
+           "qp_states": f"{lo}..{hi - 1} (HOMO = {nocc - 1}) at all {s.nk} k", "runs": {}}
+    eqp = {}
     for pn in precs:
-        h.set_precision(L.PREC[pn], L.PREC[pn])
+        # chi0 and Sigma_c contractions at pn (mixed: chi0 in the mixed mode, Sigma_c in FP64)
+        h.set_precision(L.PREC[pn], L.FP64 if pn == "mixed" else L.PREC[pn])
         L.profile_reset()
         t0 = time.perf_counter()
         h.build_g0w0_sigma(o)
         t = time.perf_counter() - t0
         prof = L.profile()
-        res["runs"][pn] = {"wall_s": round(t, 3), "regions_ms": {k: round(v[0], 2) for k, v in prof.items()},
-                           "regions_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0
-                                              for k, v in prof.items()}}
-        _log(f"{pn}: G0W0 build {t:.2f} s")
-    res["value"] = res["runs"][precs[0]]["wall_s"]
+        sc = h.get_g0w0_sigc_kgrid(o, 1, iks, lo, hi, zeros, zeros)
+        e_ks = s.eps[0][:, lo:hi].reshape(-1)
+        eqp[pn] = (e_ks + sc.real) * 27.211386245988
+        run = {"wall_s": round(t, 3), "regions_ms": {k: round(v[0], 2) for k, v in prof.items()},
+               "regions_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0
+                                  for k, v in prof.items()},
+               "qp_all_finite": bool(np.isfinite(eqp[pn]).all())}
+        if pn != "fp64":
+            d = np.abs(eqp[pn] - eqp["fp64"]) * 1e3
+            run["max_abs_dEqp_meV"] = round(float(np.nanmax(d)), 4)
+            run["homo_lumo_dEqp_meV"] = [round(float(np.nanmax(d.reshape(s.nk, nst)[:, 3])), 4),
+                                         round(float(np.nanmax(d.reshape(s.nk, nst)[:, 4])), 4)]
+        res["runs"][pn] = run
+        _log(f"{pn}: G0W0 build {t:.2f} s; QP " + (f"max |dE| vs fp64 {run.get('max_abs_dEqp_meV')} meV"
+                                                   if pn != "fp64" else f"HOMO..LUMO at k0 {eqp[pn][3]:.4f} "
+                                                                            f"{eqp[pn][4]:.4f} eV"))
+    res["value"] = res["runs"]["fp64"]["wall_s"]
+    res["eqp_fp64_ev"] = [round(float(x), 6) for x in eqp["fp64"]]
+    print(json.dumps(res), flush=True)
+
+
+def screen_bench(args):
+    """Block-norm screening on the 256-atom slab (8x8x4 atoms, 20 bohr vacuum, 2x2x1 k, nfreq 16).
+
+    The MLP is trained on the block norms of the 128-atom cell's unscreened FP64 build (a different
+    system), then the slab is built with the norm-product bound (this first slab run also tunes
+    the tiles), unscreened (the reference) and with the learned screener.  chi0 is FP64 in all
+    three; the Ec transforms run in FP32 in all three (log-determinant FP64), so dEc isolates the
+    screening.  The slab's chi0(R, tau) store does not fit beside its W slabs and goes through
+    pinned host memory."""
+    import gc
+
+    import librpa_amd as L
+    from librpa_amd import synth
+    L.init_global()
+    dev = None
+    try:
+        import torch
+        dev = "cuda" if torch.cuda.is_available() else None
+    except Exception:
+        torch = None
+    t0 = time.time()
+    s2 = synth.cubic128(vq_device=dev)
+    h2 = L.Handler()
+    L.load_system(h2, s2)
+    o2 = L.Options(nfreq=s2.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    h2.set_precision(L.FP64, L.FP64)
+    h2.set_keep_chi0(True)
+    h2.get_rpa_correlation_energy(o2, s2.nk)
+    X, t = h2.get_screen_samples()
+    model, loss = L.screen_train(X, t, iters=1500, lr=3e-3, seed=0)
+    _log(f"trained on cubic128: {len(t)} samples, standardised mse {loss:.2e} ({time.time() - t0:.0f} s)")
+    del h2, s2, X, t
+    gc.collect()
+    wl = args.workload if args.workload != "cubic128" else "slab256"
+    t0 = time.time()
+    s = synth.WORKLOADS[wl](vq_device=dev)
+    if torch is not None and dev:
+        torch.cuda.empty_cache()
+    h = L.Handler()
+    L.load_system(h, s)
+    s.vq = None  # the library holds its own copy
+    gc.collect()
+    _log(f"workload {wl}: {s.natom} atoms, n_ao {s.nbasis}, n_abf {s.nabf_tot}, nk {s.nk}, nfreq {s.nfreq} "
+         f"(generated and set in {time.time() - t0:.0f} s)")
+    o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=L.LIBRI)
+    h.set_precision(L.FP64, L.FP32)
+    thr = 1e-8
+    res = {"metric": "chi0+RPA Ec with block-norm screening", "unit": "s", "n_gpus": 1, "threshold": thr,
+           "config": {"workload": wl, "atoms": s.natom, "n_ao": s.nbasis, "n_abf": s.nabf_tot, "nk": s.nk,
+                      "nfreq": s.nfreq}, "chi0_precision": "fp64", "ec_transforms": "fp32 (log-det fp64)",
+           "model_trained_on": "cubic128 unscreened FP64 block norms", "train_mse_standardised": float(loss),
+           "data": "synthetic (seeded generator, librpa_amd/synth.py)", "runs": {}}
+    for name, mode, m in (("bound", L.SCREEN_BOUND, None), ("unscreened", L.SCREEN_OFF, None),
+                          ("learned", L.SCREEN_LEARNED, model)):
+        h.set_screening(mode, thr, m)
+        _sync()
+        t0 = time.perf_counter()
+        ec, _ = h.get_rpa_correlation_energy(o, s.nk)
+        _sync()
+        wall = time.perf_counter() - t0
+        st, sc = h.get_stats(), h.get_screen_stats()
+        res["runs"][name] = {"ec_ha": ec, "wall_s": round(wall, 2), "t_chi0_s": round(st["t_chi0"], 2),
+                             "t_ec_s": round(st["t_ec"], 2),
+                             "skipped_block_fraction": round(sc["skipped"] / max(1.0, sc["blocks"]), 6),
+                             "skipped_flop_fraction": round(sc["flops_skipped"] / max(1.0, sc["flops"]), 6)}
+        _log(f"{name}: Ec {ec:.10f}  wall {wall:.1f} s (chi0 {st['t_chi0']:.1f} s, Ec {st['t_ec']:.1f} s), "
+             f"skipped blocks {res['runs'][name]['skipped_block_fraction']}")
+    ref = res["runs"]["unscreened"]
+    for name in ("bound", "learned"):
+        r = res["runs"][name]
+        r["dEc_per_atom_ha"] = (r["ec_ha"] - ref["ec_ha"]) / s.natom
+        r["chi0_speedup_vs_unscreened"] = round(ref["t_chi0_s"] / r["t_chi0_s"], 3) if r["t_chi0_s"] > 0 else None
+    res["value"] = ref["wall_s"]
+    res["note"] = "the bound run is the slab's first and includes tile tuning"
     print(json.dumps(res), flush=True)
 
 
@@ -252,8 +364,9 @@ def main():
     ap.add_argument("--no-screen", action="store_true", help="skip the block-norm screening report")
     ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
                     help="where the synthetic Coulomb matrix product is formed (cpu under profilers)")
-    ap.add_argument("--task", default="rpa", choices=["rpa", "g0w0"],
-                    help="rpa: the chi0 + Ec metric (default); g0w0: W and Sigma_c timing on --workload")
+    ap.add_argument("--task", default="rpa", choices=["rpa", "g0w0", "screen"],
+                    help="rpa: the chi0 + Ec metric (default); g0w0: W and Sigma_c timing and QP precision "
+                         "on --workload; screen: block-norm screening on the 256-atom slab")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -266,6 +379,9 @@ def main():
         return
     if args.task == "g0w0":
         g0w0_bench(args)
+        return
+    if args.task == "screen":
+        screen_bench(args)
         return
 
     import librpa_amd as L

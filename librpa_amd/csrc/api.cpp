@@ -511,7 +511,8 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
             eng->screen_mode = d.screen_mode;
             eng->screen_thr = d.screen_thr;
             eng->screen_model = d.screen_model;
-            eng->collect = d.keep_chi0 && d.screen_mode == 0;
+            // training samples need the chi0 store on the device (not kept when it is host-staged)
+            eng->collect = d.keep_chi0 && d.screen_mode == 0 && !eng->host_staged;
             eng->build_chi0(d, d.grid, o.n_bands_chi0);
             auto r = eng->ec(d, d.grid);
             if ((int)r.qcontrib.size() != n_ibz_kpoints && o.output_level >= LIBRPA_VERBOSE_WARN)
@@ -624,6 +625,8 @@ int librpa_amd_get_screen_samples(LibrpaHandler *h, float *X, float *t) {
     return guarded(
         [&]() -> int {
             auto &d = ds_of(h);
+            if (d.rpa && d.rpa->host_staged)
+                LRA_THROW("no samples: this system's chi0 store is host-staged (too large for the device)");
             if (!d.rpa || d.rpa->features.empty())
                 LRA_THROW("no samples: build chi0 with librpa_amd_set_keep_chi0(h, 1) and screening off");
             auto &e = *d.rpa;

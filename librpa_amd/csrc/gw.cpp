@@ -496,6 +496,7 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     const int n = NABF;
     const int nw = g.n;
     if (R0.ntau_done != g.n) LRA_THROW("chi0 must be built on the same grid before W");
+    if (R0.host_staged) LRA_THROW("G0W0 needs the device-resident chi0 store (too large for this device)");
     d_Wrt.alloc(sizeof(double) * (size_t)g.n * nR * n * n);
     HIP_CHECK(hipMemsetAsync(d_Wrt.get(), 0, d_Wrt.bytes(), st));
     // no truncated Coulomb: v_c^{1/2} = 0 and W = 0 (epsilon.cpp:1862-1879)

@@ -53,6 +53,10 @@ void launch_tau_to_omega(const double *chi0s, long long E_tot, int ntau, const d
                          hipStream_t s);
 void launch_tau_to_omega(const float *chi0s, long long E_tot, int ntau, const double *coef, int nw, double *chiw,
                          hipStream_t s);
+// chiw[w][i] = sum_{t' <= t} coef[w][t'] slice_t'[i], one slice per call (t = 0 overwrites)
+template <class TS>
+void launch_tau_accumulate(const TS *slice, long long E_tot, int t, int ntau, const double *coef, int nw, double *chiw,
+                           hipStream_t s);
 template <class TD>
 void launch_chi0_rq(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs, long long n_pair_elems,
                     const long long *eoff, int nR, const double *ph, int nw, TD *dense, long long dense_stride, int n,

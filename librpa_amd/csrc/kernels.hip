@@ -244,6 +244,35 @@ void launch_tau_to_omega(const float *chi0s, long long E_tot, int ntau, const do
     HIP_CHECK(hipGetLastError());
 }
 
+// chiw[w][i] (+)= coef[w][t] slice[i] for one tau slice t (the first slice overwrites): the same
+// sums as tau_to_omega_kernel, accumulated slice by slice in the same order of t
+template <class TS>
+__global__ __launch_bounds__(256) void tau_accumulate_kernel(const TS *__restrict__ slice, long long E_tot, int t,
+                                                             int ntau, const double *__restrict__ coef, int nw,
+                                                             double *__restrict__ chiw) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < E_tot;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double v = static_cast<double>(slice[i]);
+        for (int w = 0; w < nw; ++w) {
+            double *d = chiw + (long long)w * E_tot + i;
+            const double base = t == 0 ? 0.0 : *d;
+            *d = base + coef[w * ntau + t] * v;
+        }
+    }
+}
+
+template <class TS>
+void launch_tau_accumulate(const TS *slice, long long E_tot, int t, int ntau, const double *coef, int nw, double *chiw,
+                           hipStream_t s) {
+    const int bx = (int)std::min<long long>(ceil_div(E_tot, 256), 16384);
+    hipLaunchKernelGGL(tau_accumulate_kernel<TS>, dim3(bx), dim3(256), 0, s, slice, E_tot, t, ntau, coef, nw, chiw);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_tau_accumulate<double>(const double *, long long, int, int, const double *, int, double *,
+                                            hipStream_t);
+template void launch_tau_accumulate<float>(const float *, long long, int, int, const double *, int, double *,
+                                           hipStream_t);
+
 // dense chi0(q, omega) of nw frequencies from chi0(R, omega): per pair element and frequency,
 // sum_r (cos_r + i sin_r) chiw[w][eoff[p][r] + e], written like chi0_assemble (Hermitian fill,
 // real or embedded).  Reads chiw once per (element, R, frequency).

@@ -182,7 +182,13 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     // ---- chunking over J by memory budget (gathered G panels and T) ----
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    double budget = std::min(48e9, 0.35 * (double)free_b - (double)w_off.back() * ts);
+    // chi0(R, tau) of every tau beside the W slabs, or staged through pinned host memory
+    const double w_bytes = (double)w_off.back() * ts, slice_bytes = (double)ts * E_tot;
+    free_host_store();
+    host_staged = w_bytes + slice_bytes * ntau + 24e9 > (double)free_b;
+    if (const char *hs = std::getenv("LIBRPA_AMD_HOST_CHI0")) host_staged = host_staged || std::atoi(hs) != 0;
+    double budget = host_staged ? std::min(48e9, 0.5 * ((double)free_b - w_bytes - slice_bytes - 16e9))
+                                : std::min(48e9, 0.35 * (double)free_b - w_bytes);
     // LIBRPA_AMD_CHUNK_BYTES=<bytes> caps the chunk size (smaller chunks, same results)
     if (const char *cb = std::getenv("LIBRPA_AMD_CHUNK_BYTES")) {
         const double v = std::atof(cb);
@@ -434,7 +440,11 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     }
 
     // ---- chi0s store and transform descriptors ----
-    d_chi0s.alloc(ts * (size_t)E_tot * ntau);
+    d_chi0s.alloc(ts * (size_t)E_tot * (host_staged ? 1 : ntau));
+    if (host_staged) {
+        h_chi0s.assign(ntau, nullptr);
+        for (int t = 0; t < ntau; ++t) HIP_CHECK(hipHostMalloc(&h_chi0s[t], ts * (size_t)E_tot, hipHostMallocDefault));
+    }
     pairs.clear();
     pair_eoff.clear();
     n_pair_elems = 0;
@@ -543,6 +553,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     }
     HIP_CHECK(hipMemsetAsync(d_chi0s.get(), 0, d_chi0s.bytes(), stream));
     if (screen_mode != 0 && nspin > 1) LRA_THROW("chi0 block screening needs nspin = 1");
+    if (host_staged && collect) LRA_THROW("screening samples need the device-resident chi0 store (too large here)");
     if (screen_mode == 2) {
         if (screen_model.size() != (size_t)MLP_NMODEL) LRA_THROW("learned screening without a model");
         d_model = upload(screen_model, stream);
@@ -583,7 +594,8 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         for (int s = 0; s < nspin; ++s) {
             build_gf(mf, tau, s);
             if (s == 0 && (screen_mode != 0 || collect)) screen_tau(it, tau);
-            char *chi_base = lowp ? d_chi32.as<char>() : d_chi0s.as<char>() + (size_t)it * E_tot * ts;
+            char *chi_base =
+                lowp ? d_chi32.as<char>() : d_chi0s.as<char>() + (host_staged ? 0 : (size_t)it * E_tot * ts);
             // tiles are tuned per precision on the first tau point built at it
             const bool tune_now = autotune && s == 0 && !tuned_chi0[lowp];
             for (auto &c : chunks) {
@@ -657,7 +669,11 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         tuned_chi0[lowp] = tuned_chi0[lowp] || autotune;
         // the FP32 result of a low-precision tau point goes into the FP64 chi0(R, tau) store
         if (lowp)
-            launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (size_t)it * E_tot, E_tot, stream);
+            launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (host_staged ? 0 : (size_t)it * E_tot),
+                                          E_tot, stream);
+        // a host-staged build keeps its slice in pinned memory (the next tau waits on the copy)
+        if (host_staged)
+            HIP_CHECK(hipMemcpyAsync(h_chi0s[it], d_chi0s.get(), ts * (size_t)E_tot, hipMemcpyDeviceToHost, stream));
     }
     cur = prec;  // later G builds (the self-energy's) run at the plan's precision
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -702,7 +718,15 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     if (!chiw_ok) {
         prof.begin("chi0_tau_to_omega", stream);
-        form_chiw(g, 0.33 * (double)free_b, stream);
+        if (host_staged) {
+            // the chi0 work buffers make room for chi0(R, omega); build_chi0 reserves them again
+            release_work();
+            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+            if (!form_chiw(g, 0.8 * (double)free_b, stream))
+                LRA_THROW("chi0(R, omega) does not fit on the device beside the Ec buffers");
+        } else {
+            form_chiw(g, 0.33 * (double)free_b, stream);
+        }
         prof.end("chi0_tau_to_omega", stream, 0, (double)E_tot * g.n * (ts * ceil_div(g.n, 8) + 8.0));
     }
     // all omega of a q are factorised together, as far as memory allows
@@ -858,6 +882,21 @@ bool RpaEngine::form_chiw(const TFGrid &g, double max_bytes, hipStream_t s) {
     for (int w = 0; w < g.n; ++w)
         for (int t = 0; t < g.n; ++t) coef[(size_t)w * g.n + t] = g.cos_t2f[(size_t)w * g.n + t] * 2.0 / nspin;
     DevBuf dc = upload(coef, s);
+    if (host_staged) {
+        // slices back through the one-slice device buffer, accumulated tau by tau
+        for (int t = 0; t < ntau_done; ++t) {
+            HIP_CHECK(hipMemcpyAsync(d_chi0s.get(), h_chi0s[t], ts * (size_t)E_tot, hipMemcpyHostToDevice, s));
+            if (prec == Prec::F64)
+                launch_tau_accumulate(d_chi0s.as<double>(), E_tot, t, ntau_done, dc.as<double>(), g.n,
+                                      d_chiw.as<double>(), s);
+            else
+                launch_tau_accumulate(d_chi0s.as<float>(), E_tot, t, ntau_done, dc.as<double>(), g.n,
+                                      d_chiw.as<double>(), s);
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+        chiw_ok = true;
+        return true;
+    }
     if (prec == Prec::F64)
         launch_tau_to_omega(d_chi0s.as<double>(), E_tot, ntau_done, dc.as<double>(), g.n, d_chiw.as<double>(), s);
     else
@@ -928,7 +967,14 @@ void RpaEngine::release_work() {
     d_T2.release();
 }
 
+void RpaEngine::free_host_store() {
+    for (void *p : h_chi0s)
+        if (p) (void)hipHostFree(p);
+    h_chi0s.clear();
+}
+
 RpaEngine::~RpaEngine() {
+    free_host_store();
     if (s2) {
         (void)hipStreamSynchronize(s2);
         for (hipEvent_t e : {ev_ready, ev_s2, ev_ta[0], ev_ta[1], ev_c[0], ev_c[1]})
@@ -942,6 +988,16 @@ void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {
         if (o.I == I && o.J == J && o.r == r) {
             const size_t n = (size_t)na[I] * na[J];
             const size_t off = (size_t)it * E_tot + o.eoff;
+            if (host_staged) {
+                if (it < 0 || it >= (int)h_chi0s.size()) LRA_THROW("tau index out of range");
+                if (prec == Prec::F64) {
+                    std::memcpy(out, static_cast<const double *>(h_chi0s[it]) + o.eoff, n * 8);
+                } else {
+                    const float *src = static_cast<const float *>(h_chi0s[it]) + o.eoff;
+                    for (size_t i = 0; i < n; ++i) out[i] = src[i];
+                }
+                return;
+            }
             if (prec == Prec::F64) {
                 HIP_CHECK(hipMemcpy(out, d_chi0s.as<double>() + off, n * 8, hipMemcpyDeviceToHost));
             } else {
@@ -956,6 +1012,7 @@ void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {
 
 void RpaEngine::chi0q_block(const Dataset &ds, const TFGrid &g, int iw, int iq, int I, int J, double *re,
                             double *im) {
+    if (host_staged) LRA_THROW("chi0(q, omega) blocks are not kept with the host-staged chi0 store");
     const bool embed = nk > 1;
     const int n = NABF, n2 = embed ? 2 * NABF : NABF;
     DevBuf dD((size_t)n2 * n2 * 8);

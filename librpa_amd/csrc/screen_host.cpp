@@ -162,6 +162,7 @@ void RpaEngine::screen_tau(int it, double tau) {
 }
 
 void RpaEngine::chi0_norms(std::vector<double> &out) {
+    if (host_staged) LRA_THROW("chi0 block norms need the device-resident chi0 store (too large here)");
     const int nout = (int)outs.size();
     out.assign((size_t)ntau_done * nout, 0.0);
     if (out.empty()) return;

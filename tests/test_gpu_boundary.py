@@ -27,7 +27,7 @@ def _say(msg):
 def _ec(s, routing=L.LIBRI, opt_routing=L.LIBRI, keep=False, prec=L.FP64):
     h = L.Handler()
     L.load_system(h, s, routing=routing)
-    h.set_precision(prec, prec)
+    h.set_precision(prec, L.FP64 if prec == L.MIXED else prec)
     if keep:
         h.set_keep_chi0(True)
     o = L.Options(nfreq=s.nfreq, tfgrids_type=L.MINIMAX, parallel_routing=opt_routing)
@@ -148,3 +148,23 @@ def test_split_k_parts_agree_and_single_part_is_bitwise_repeatable(monkeypatch):
     _say(f"  Ec split 1 {ec1!r}  split 4 {ec4!r}  chi0 rel diff {worst:.1e}")
     assert worst < 1e-13
     assert abs(ec4 - ec1) < 1e-12 * abs(ec1)
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
+def test_host_staged_chi0_store_gives_the_same_result(monkeypatch, prec):
+    # chi0(R, tau) kept in pinned host memory slice by slice (what a store too large for the
+    # device does) and streamed back into chi0(R, omega) for Ec: the same Ec and blocks
+    _say(f"host-staged chi0 store, prec {prec}")
+    s = synth.cell2()
+    taus = (0, 7, 15)
+    monkeypatch.setenv("LIBRPA_AMD_SPLITK", "1")
+    h0, _, ec0, c0 = _ec(s, keep=True, prec=prec)
+    b0 = _chi0_blocks(h0, s, taus)
+    monkeypatch.setenv("LIBRPA_AMD_HOST_CHI0", "1")
+    h1, _, ec1, c1 = _ec(s, keep=True, prec=prec)
+    b1 = _chi0_blocks(h1, s, taus)
+    worst = max(np.abs(b1[k] - b0[k]).max() / max(np.abs(b0[k]).max(), 1e-300) for k in b0)
+    _say(f"  Ec device store {ec0!r}, host-staged {ec1!r}; chi0 blocks max rel diff {worst:.1e}")
+    assert worst == 0.0
+    assert abs(ec1 - ec0) <= 1e-13 * abs(ec0)
+    assert np.abs(c1 - c0).max() <= 1e-13 * abs(ec0)
