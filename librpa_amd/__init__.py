@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys as _sys
 from typing import Optional
 
 import numpy as np
@@ -468,7 +469,7 @@ class Handler:
         return dict(zip(keys, out.tolist()))
 
 
-def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut=False):
+def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut=False, verbose: bool = False):
     """Push a librpa_amd.synth.System through the C API (mirrors the driver's read_* calls).
 
     ``vq_cut``: True passes the Coulomb matrix again as the truncated one used for W (G0W0);
@@ -490,6 +491,8 @@ def load_system(h: Handler, sys, routing: int = LIBRI, vq: bool = True, vq_cut=F
     if vq and sys.vq is not None:
         off = np.concatenate([[0], np.cumsum(sys.nabf)])
         for ik in range(nk):
+            if verbose:
+                print(f"[load_system] Coulomb blocks of k {ik + 1} / {nk}", file=_sys.stderr, flush=True)
             V = sys.vq[ik]
             for I in range(sys.natom):
                 for J in range(I, sys.natom):

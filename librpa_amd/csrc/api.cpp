@@ -505,9 +505,11 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
                 d.gw.reset();
                 eng = std::make_shared<RpaEngine>();
                 eng->mixed = d.chi0_mixed;
+                eng->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
                 eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
             }
             eng->ec_prec = d.ec_prec;
+            eng->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
             eng->screen_mode = d.screen_mode;
             eng->screen_thr = d.screen_thr;
             eng->screen_model = d.screen_model;

@@ -230,7 +230,14 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     std::set<int> classes(nb.begin(), nb.end());
     std::map<std::tuple<int, int, int, int>, int> kt_pos;  // K-table start per species pair (per chunk)
 
+    auto say = [&](const char *what, size_t a, size_t b) {
+        if (!verbose) return;
+        fprintf(stderr, "[librpa-mi355x] plan: %s %zu / %zu\n", what, a, b);
+        fflush(stderr);
+    };
+    say("outputs (I <= J, R) and chunks", outs.size(), chunks.size());
     for (auto &c : chunks) {
+        say("problem lists of chunk", (size_t)(&c - chunks.data()) + 1, chunks.size());
         kt_pos.clear();
         // offsets inside the chunk buffers
         long long go = 0, to = 0;
@@ -442,8 +449,9 @@ void RpaEngine::plan(const Dataset &ds, Prec p, Prec pe, int ntau) {
     // ---- chi0s store and transform descriptors ----
     d_chi0s.alloc(ts * (size_t)E_tot * (host_staged ? 1 : ntau));
     if (host_staged) {
-        h_chi0s.assign(ntau, nullptr);
-        for (int t = 0; t < ntau; ++t) HIP_CHECK(hipHostMalloc(&h_chi0s[t], ts * (size_t)E_tot, hipHostMallocDefault));
+        // pageable memory: a store this size can exceed what the host lets a process pin
+        h_chi0s.assign(ntau, {});
+        for (int t = 0; t < ntau; ++t) h_chi0s[t].resize(ts * (size_t)E_tot);
     }
     pairs.clear();
     pair_eoff.clear();
@@ -529,6 +537,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     const double t0 = now_s();
     const int ntau = g.n;
     chiw_ok = false;  // chi0(R, omega) of an earlier build is stale
+    if (host_staged) d_chiw.release();  // its room goes back to W and T (reformed for Ec)
     prepare_gf(mf, nbands_g);
     d_Ghat.reserve(ghat_max * ts);
     d_W.reserve((size_t)w_off.back() * ts);
@@ -540,6 +549,8 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     }
     cur = prec;
     if (const char *ov = std::getenv("LIBRPA_AMD_OVERLAP")) overlap_ta = std::atoi(ov) != 0;
+    // LIBRPA_AMD_AUTOTUNE=0 keeps the default tile shapes (no timing runs on first use)
+    if (const char *at = std::getenv("LIBRPA_AMD_AUTOTUNE")) autotune = std::atoi(at) != 0;
     if (overlap_ta && chunks.size() > 1) {
         if (!s2) {
             HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
@@ -599,6 +610,11 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
             // tiles are tuned per precision on the first tau point built at it
             const bool tune_now = autotune && s == 0 && !tuned_chi0[lowp];
             for (auto &c : chunks) {
+                if (tune_now && verbose) {
+                    fprintf(stderr, "[librpa-mi355x] chi0: tuning the W build of chunk %d / %zu\n",
+                            (int)(&c - chunks.data()) + 1, chunks.size());
+                    fflush(stderr);
+                }
                 prof.begin(rn("chi0_gather"), stream);
                 if (cur == Prec::F64)
                     launch_gather_cols<double>(d_Gpm.as<double>(), 2LL * NAO, d_Ghat.as<double>(), c.d_jobs.as<GatherJob>(),
@@ -644,6 +660,11 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 continue;
             }
             for (auto &c : chunks) {
+                if (tune_now && verbose) {
+                    fprintf(stderr, "[librpa-mi355x] chi0: tuning T_A and the contraction of chunk %d / %zu\n",
+                            (int)(&c - chunks.data()) + 1, chunks.size());
+                    fflush(stderr);
+                }
                 if (tune_now && c.ta.cfg_by_prec[(int)cur] < 0) {
                     // TA overwrites T; the contraction trials accumulate into chi0s before the X
                     // term below resets it (beta = 0) for this tau
@@ -667,13 +688,19 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
             }
         }
         tuned_chi0[lowp] = tuned_chi0[lowp] || autotune;
+        if (verbose) {
+            HIP_CHECK(hipStreamSynchronize(stream));
+            fprintf(stderr, "[librpa-mi355x] chi0: tau %d / %d (%.1f s)\n", it + 1, ntau, now_s() - t0);
+            fflush(stderr);
+        }
         // the FP32 result of a low-precision tau point goes into the FP64 chi0(R, tau) store
         if (lowp)
             launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (host_staged ? 0 : (size_t)it * E_tot),
                                           E_tot, stream);
         // a host-staged build keeps its slice in pinned memory (the next tau waits on the copy)
         if (host_staged)
-            HIP_CHECK(hipMemcpyAsync(h_chi0s[it], d_chi0s.get(), ts * (size_t)E_tot, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(h_chi0s[it].data(), d_chi0s.get(), ts * (size_t)E_tot, hipMemcpyDeviceToHost,
+                                     stream));
     }
     cur = prec;  // later G builds (the self-energy's) run at the plan's precision
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -791,6 +818,11 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     };
     for (int iq = 0; iq < nq; ++iq) {
         const int kq = ibz[iq];
+        if (verbose) {
+            HIP_CHECK(hipStreamSynchronize(stream));
+            fprintf(stderr, "[librpa-mi355x] Ec: q %d / %d (%.1f s)\n", iq + 1, nq, now_s() - t0);
+            fflush(stderr);
+        }
         // dense V(q) from the staged upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
         prof.begin("ec_v_fill", stream);
         HIP_CHECK(hipMemsetAsync(dV.get(), 0, nn * 8, stream));
@@ -885,7 +917,7 @@ bool RpaEngine::form_chiw(const TFGrid &g, double max_bytes, hipStream_t s) {
     if (host_staged) {
         // slices back through the one-slice device buffer, accumulated tau by tau
         for (int t = 0; t < ntau_done; ++t) {
-            HIP_CHECK(hipMemcpyAsync(d_chi0s.get(), h_chi0s[t], ts * (size_t)E_tot, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(d_chi0s.get(), h_chi0s[t].data(), ts * (size_t)E_tot, hipMemcpyHostToDevice, s));
             if (prec == Prec::F64)
                 launch_tau_accumulate(d_chi0s.as<double>(), E_tot, t, ntau_done, dc.as<double>(), g.n,
                                       d_chiw.as<double>(), s);
@@ -968,9 +1000,8 @@ void RpaEngine::release_work() {
 }
 
 void RpaEngine::free_host_store() {
-    for (void *p : h_chi0s)
-        if (p) (void)hipHostFree(p);
     h_chi0s.clear();
+    h_chi0s.shrink_to_fit();
 }
 
 RpaEngine::~RpaEngine() {
@@ -991,9 +1022,9 @@ void RpaEngine::chi0s_block(int it, int I, int J, int r, double *out) {
             if (host_staged) {
                 if (it < 0 || it >= (int)h_chi0s.size()) LRA_THROW("tau index out of range");
                 if (prec == Prec::F64) {
-                    std::memcpy(out, static_cast<const double *>(h_chi0s[it]) + o.eoff, n * 8);
+                    std::memcpy(out, reinterpret_cast<const double *>(h_chi0s[it].data()) + o.eoff, n * 8);
                 } else {
-                    const float *src = static_cast<const float *>(h_chi0s[it]) + o.eoff;
+                    const float *src = reinterpret_cast<const float *>(h_chi0s[it].data()) + o.eoff;
                     for (size_t i = 0; i < n; ++i) out[i] = src[i];
                 }
                 return;

@@ -111,11 +111,12 @@ struct RpaEngine {
     double screen_thr = 1e-8;     // skip blocks whose bound (or prediction) is below this norm
     std::vector<float> screen_model;  // MLP_NMODEL values (parameters, feature and target scaling)
     bool collect = false;         // record the features of every (tau, output) of a build
-    // chi0(R, tau) in pinned host memory (one pointer per tau) with one tau slice on the device,
+    // chi0(R, tau) in host memory (one buffer per tau) with one tau slice on the device,
     // when the device store would not fit beside the resident W slabs (LIBRPA_AMD_HOST_CHI0=1
     // forces it); the Ec step then streams the slices back into chi0(R, omega)
     bool host_staged = false;
-    std::vector<void *> h_chi0s;
+    bool verbose = false;  // progress lines on stderr (tau of the chi0 build, q of Ec)
+    std::vector<std::vector<char>> h_chi0s;
     void free_host_store();
     std::vector<double> cnorm;    // per atom: sum of the Frobenius norms of its folded Cs blocks
     std::vector<float> feat_static;  // per output: d / 10, n_b(I) / 27, n_b(J) / 27, ln c_I, ln c_J

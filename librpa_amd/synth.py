@@ -25,11 +25,14 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import sys
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
 CS_CUTOFF = 12.0
+# when true, the generator reports each k of the Coulomb matrices on stderr (large workloads)
+VERBOSE = False
 
 
 @dataclasses.dataclass
@@ -182,23 +185,33 @@ def build_system(name: str, latvec, positions, types, species: List[Species], pe
         scale = 1.0 / (1.0 + np.arange(n))
         for ik in range(nk):
             b = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * scale[None, :]
-            vq[ik] = _bbh(b, vq_device) / n + 1e-3 * np.eye(n)
+            vq[ik] = _bbh(b, vq_device, n, 1e-3)
             if real_gamma_v and np.all(kfrac[ik] == 0):
                 vq[ik] = vq[ik].real.astype(np.complex128)
+            if VERBOSE:
+                print(f"[synth] {name}: Coulomb matrix of k {ik + 1} / {nk}", file=sys.stderr, flush=True)
     return System(name=name, latvec=latvec, positions=positions, types=types, nao=nao, nabf=nabf,
                   period=tuple(period), kfrac=kfrac, eps=eps, occ=occ, efermi=float(efermi), wfc=wfc,
                   cs=cs, vq=vq, nfreq=nfreq)
 
 
-def _bbh(b, device):
+def _bbh(b, device, inv=1.0, shift=0.0):
+    """(0.5 (B B^H + (B B^H)^H)) / inv + shift * I, in this order of operations on both paths."""
     if device is not None:
         import torch
         t = torch.from_numpy(b).to(device)
-        r = (t @ t.conj().T).cpu().numpy()
+        r = t @ t.conj().T
         r = 0.5 * (r + r.conj().T)
-        return r
+        r = r / inv
+        r.diagonal().add_(shift)
+        out = r.cpu().numpy()
+        del t, r
+        return out
     r = b @ b.conj().T
-    return 0.5 * (r + r.conj().T)
+    r = 0.5 * (r + r.conj().T)
+    r = r / inv
+    r[np.diag_indices_from(r)] += shift
+    return r
 
 
 # ----------------------------------------------------------------------------------------------
