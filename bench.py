@@ -74,9 +74,9 @@ def screening_report(L, h, o, s, ec64, thr=1e-8):
 
 
 def run_steps(L, h, o, nk, prec, steps, warmup):
-    # the mixed mode is a chi0 setting; with it the Ec transforms (chi0 L, L^T X) run in FP32, which
+    # the mixed modes are chi0 settings; with them the Ec transforms (chi0 L, L^T X) run in FP32, which
     # the tau->omega transform does not amplify (the log-determinant stays FP64)
-    h.set_precision(prec, L.FP32 if prec == L.MIXED else prec)
+    h.set_precision(prec, L.FP32 if prec in (L.MIXED, L.MIXED_BF16) else prec)
     ec = None
     for _ in range(warmup):
         ec, _ = h.get_rpa_correlation_energy(o, nk)
@@ -108,10 +108,16 @@ def _profiled_traffic(kernel, workload, prec_name):
 def dominant_kernel(prof, prec_name, workload):
     """Largest-time region among the single-kernel GEMM regions, placed on its roofline."""
     cands = {k: v for k, v in prof.items() if k.startswith(("chi0_W", "chi0_TA", "chi0_X", "chi0_contract",
-                                                            "ec_gemm"))}
+                                                            "ec_gemm")) and v[2] > 0}
     name, (ms, calls, flops, nbytes) = max(cands.items(), key=lambda kv: kv[1][0])
-    if prec_name == "mixed":  # FP32: the tau points' regions with an _f32 suffix and the Ec GEMMs
-        prec_name = "fp32" if name.endswith("_f32") or name.startswith("ec_gemm") else "fp64"
+    # regions named for the precision they ran in (_f32, _bf16x3: low-weight tau points of the mixed
+    # modes, and the small-K GEMMs that stay in FP32 under BF16x3)
+    if name.endswith("_bf16x3"):
+        prec_name = "bf16x3"
+    elif name.endswith("_f32"):
+        prec_name = "fp32"
+    elif prec_name.startswith("mixed"):  # the plan precision, and FP32 Ec GEMMs
+        prec_name = "fp32" if name.startswith("ec_gemm") else "fp64"
     peak_f = PEAK_TFLOPS[prec_name]
     t_f = flops / (peak_f * 1e12)
     t_b = nbytes / (HBM_GBS * 1e9)
@@ -249,7 +255,7 @@ def g0w0_bench(args):
     eqp = {}
     for pn in precs:
         # chi0 and Sigma_c contractions at pn (mixed: chi0 in the mixed mode, Sigma_c in FP64)
-        h.set_precision(L.PREC[pn], L.FP64 if pn == "mixed" else L.PREC[pn])
+        h.set_precision(L.PREC[pn], L.FP64 if pn.startswith("mixed") else L.PREC[pn])
         L.profile_reset()
         t0 = time.perf_counter()
         h.build_g0w0_sigma(o)
@@ -366,7 +372,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cubic128")
-    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3", "mixed"])
+    ap.add_argument("--prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-screen", action="store_true", help="skip the block-norm screening report")
     ap.add_argument("--gen-device", default="auto", choices=["auto", "cpu"],
@@ -426,7 +432,7 @@ def main():
     _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step (median of {n64})  chi0 {st64['t_chi0']:.3f}s  "
          f"Ec {st64['t_ec']:.3f}s")
 
-    cand = ["bf16x3", "fp32", "mixed"] if args.prec == "auto" else [args.prec]
+    cand = ["bf16x3", "fp32", "mixed-bf16", "mixed"] if args.prec == "auto" else [args.prec]
     chosen = None
     for pn in cand:
         if pn == "fp64":

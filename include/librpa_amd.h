@@ -17,6 +17,9 @@ extern "C" {
  * matrix has an entry above 1e3 (the transform multiplies their chi0 errors
  * by up to that much), FP32 at the others; chi0(R, tau) is stored in FP64. */
 #define LIBRPA_AMD_MIXED 3
+/* chi0 only: as LIBRPA_AMD_MIXED with BF16x3 instead of FP32 at the low-weight tau points (their
+ * small-K X and T_A GEMMs stay FP32). */
+#define LIBRPA_AMD_MIXED_BF16 4
 
 /* Contraction precision of the chi0 build (G, W, T and the final Cs.T
  * contraction) and of the Ec transforms (chi0 L and L^T X).  The Cholesky

@@ -22,8 +22,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librpa_mi355x.so")
 _lib = None
 
-FP64, FP32, BF16X3, MIXED = 0, 1, 2, 3
-PREC = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "mixed": MIXED}
+FP64, FP32, BF16X3, MIXED, MIXED_BF16 = 0, 1, 2, 3, 4
+PREC = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "mixed": MIXED, "mixed-bf16": MIXED_BF16}
 
 # LibrpaTimeFreqGrid / LibrpaParallelRouting values (librpa_enums.h)
 MINIMAX = 3

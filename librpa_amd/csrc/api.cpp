@@ -509,6 +509,7 @@ double librpa_get_rpa_correlation_energy(LibrpaHandler *h, const LibrpaOptions *
                 eng->plan(d, d.chi0_prec, d.ec_prec, d.grid.n);
             }
             eng->ec_prec = d.ec_prec;
+            eng->mixed_low = d.chi0_mixed_low;
             eng->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
             eng->screen_mode = d.screen_mode;
             eng->screen_thr = d.screen_thr;
@@ -576,8 +577,9 @@ void librpa_get_g0w0_sigc_band_k(LibrpaHandler *h, const LibrpaOptions *, const 
 void librpa_amd_set_precision(LibrpaHandler *h, int chi0_prec, int ec_prec) {
     guarded_void([&]() {
         auto &d = ds_of(h);
-        if (chi0_prec < 0 || chi0_prec > 3 || ec_prec < 0 || ec_prec > 2) LRA_THROW("unknown precision");
-        d.chi0_mixed = chi0_prec == 3;
+        if (chi0_prec < 0 || chi0_prec > 4 || ec_prec < 0 || ec_prec > 2) LRA_THROW("unknown precision");
+        d.chi0_mixed = chi0_prec >= 3;
+        d.chi0_mixed_low = chi0_prec == 4 ? Prec::BF16X3 : Prec::F32;
         d.chi0_prec = d.chi0_mixed ? Prec::F64 : static_cast<Prec>(chi0_prec);
         d.ec_prec = static_cast<Prec>(ec_prec);
     });

@@ -67,6 +67,7 @@ struct Dataset {
     Prec chi0_prec = Prec::F64;
     Prec ec_prec = Prec::F64;
     bool chi0_mixed = false;  // FP64 chi0 plan with FP32 contractions at the tau points of small weight
+    Prec chi0_mixed_low = Prec::F32;  // the precision of those tau points (FP32 or BF16x3)
     bool keep_chi0 = false;
     int screen_mode = 0;
     double screen_thr = 1e-8;

@@ -13,7 +13,8 @@
 //   F32    v_mfma_f32_16x16x4_f32   (A, B, C in float)
 //   BF16X3 v_mfma_f32_16x16x32_bf16 three times per K slice with operands split
 //          x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); acc += lo*hi + hi*lo + hi*hi,
-//          operands read as float, accumulation in float.
+//          operands read as float and split once when a tile is staged (LDS holds bf16 hi and
+//          lo planes), accumulation in float.
 #include "gemm.h"
 
 #include <algorithm>
@@ -45,16 +46,6 @@ __device__ __forceinline__ unsigned short f2bf_rne(float x) {
     return static_cast<unsigned short>(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
-
-__device__ __forceinline__ void split8(const float *x, v8s &hi, v8s &lo) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        unsigned short h = f2bf_rne(x[i]);
-        float r = x[i] - bf2f(h);
-        hi[i] = static_cast<short>(h);
-        lo[i] = static_cast<short>(f2bf_rne(r));
-    }
-}
 
 __device__ __forceinline__ int find_prob(const GemmProb *__restrict__ probs, int nprob, int t) {
     int lo = 0, hi = nprob - 1;
@@ -104,8 +95,12 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
     constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
     static_assert(LA * NT == BM * BK && LB * NT == BN * BK, "tile load split");
 
-    __shared__ T As[BM * LDK];
-    __shared__ T Bs[BN * LDK];
+    // MODE 2 stages bf16 hi / lo planes instead (LDK2 keeps 16-byte rows for the fragment loads)
+    constexpr int LDK2 = BK + 8;
+    __shared__ T As[MODE == 2 ? 1 : BM * LDK];
+    __shared__ T Bs[MODE == 2 ? 1 : BN * LDK];
+    __shared__ __align__(16) unsigned short Ahl[MODE == 2 ? 2 * BM * LDK2 : 8];
+    __shared__ __align__(16) unsigned short Bhl[MODE == 2 ? 2 * BN * LDK2 : 8];
 
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= total) return;
@@ -182,10 +177,25 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
         }
     };
     auto sstore = [&]() {
+        if constexpr (MODE == 2) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) As[am[i] * LDK + ak[i]] = ra[i];
+            for (int i = 0; i < LA; ++i) {
+                const unsigned short h = f2bf_rne(ra[i]);
+                Ahl[am[i] * LDK2 + ak[i]] = h;
+                Ahl[BM * LDK2 + am[i] * LDK2 + ak[i]] = f2bf_rne(ra[i] - bf2f(h));
+            }
 #pragma unroll
-        for (int i = 0; i < LB; ++i) Bs[bn[i] * LDK + bk[i]] = rb[i];
+            for (int i = 0; i < LB; ++i) {
+                const unsigned short h = f2bf_rne(rb[i]);
+                Bhl[bn[i] * LDK2 + bk[i]] = h;
+                Bhl[BN * LDK2 + bn[i] * LDK2 + bk[i]] = f2bf_rne(rb[i] - bf2f(h));
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < LA; ++i) As[am[i] * LDK + ak[i]] = ra[i];
+#pragma unroll
+            for (int i = 0; i < LB; ++i) Bs[bn[i] * LDK + bk[i]] = rb[i];
+        }
     };
 
     const int rowA = wm * WTM + (lane & 15);
@@ -227,23 +237,15 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
                 v8s ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
                 for (int a = 0; a < FM; ++a) {
-                    const float *p = &As[(rowA + a * 16) * LDK + kk + kq * 8];
-                    float x[8];
-                    const v4f x0 = *reinterpret_cast<const v4f *>(p);
-                    const v4f x1 = *reinterpret_cast<const v4f *>(p + 4);
-                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2]; x[3] = x0[3];
-                    x[4] = x1[0]; x[5] = x1[1]; x[6] = x1[2]; x[7] = x1[3];
-                    split8(x, ah[a], al[a]);
+                    const int o = (rowA + a * 16) * LDK2 + kk + kq * 8;
+                    ah[a] = *reinterpret_cast<const v8s *>(&Ahl[o]);
+                    al[a] = *reinterpret_cast<const v8s *>(&Ahl[BM * LDK2 + o]);
                 }
 #pragma unroll
                 for (int b = 0; b < FN; ++b) {
-                    const float *p = &Bs[(rowB + b * 16) * LDK + kk + kq * 8];
-                    float x[8];
-                    const v4f x0 = *reinterpret_cast<const v4f *>(p);
-                    const v4f x1 = *reinterpret_cast<const v4f *>(p + 4);
-                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2]; x[3] = x0[3];
-                    x[4] = x1[0]; x[5] = x1[1]; x[6] = x1[2]; x[7] = x1[3];
-                    split8(x, bh[b], bl[b]);
+                    const int o = (rowB + b * 16) * LDK2 + kk + kq * 8;
+                    bh[b] = *reinterpret_cast<const v8s *>(&Bhl[o]);
+                    bl[b] = *reinterpret_cast<const v8s *>(&Bhl[BN * LDK2 + o]);
                 }
 #pragma unroll
                 for (int a = 0; a < FM; ++a)

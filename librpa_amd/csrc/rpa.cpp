@@ -545,6 +545,7 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
     if (mixed) {
         if (prec != Prec::F64) LRA_THROW("mixed chi0 precision needs the FP64 plan");
         if (const char *e = std::getenv("LIBRPA_AMD_MIXED_AMAX")) mixed_amax = std::atof(e);
+        // the low-weight tau points run in mixed_low, FP32 or BF16x3 (both read the FP32 copies)
         d_chi32.reserve(sizeof(float) * (size_t)E_tot);
     }
     cur = prec;
@@ -585,12 +586,19 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         if (mixed) {
             double amax = 0;
             for (int w = 0; w < g.n; ++w) amax = std::max(amax, std::fabs(g.cos_t2f[(size_t)w * g.n + it]));
-            if (amax <= mixed_amax) cur = Prec::F32;
+            if (amax <= mixed_amax) cur = mixed_low;
         }
         const bool lowp = cur != prec;
         n_lowp_tau += lowp ? 1 : 0;
+        // the small-K GEMMs (X: K = 2 n_b(I) n_b(J) in a K table; T_A: K = 2 n_b(J)) stay in FP32 when
+        // the tau point runs in BF16x3, whose 32-deep K slices would pad them and cost more than FP32
+        const Prec small = cur == Prec::BF16X3 ? Prec::F32 : cur;
         // low-precision tau points time under their own region names
-        auto rn = [lowp](const char *r) { return lowp ? std::string(r) + "_f32" : std::string(r); };
+        const char *lsuf = cur == Prec::BF16X3 ? "_bf16x3" : "_f32";
+        auto rn = [lowp, lsuf](const char *r) { return lowp ? std::string(r) + lsuf : std::string(r); };
+        auto rs = [&rn, small, cur_ = cur](const char *r) {  // regions of the small-K GEMMs
+            return small == cur_ ? rn(r) : std::string(r) + "_f32";
+        };
         const double bsc = lowp ? 4.0 / ts : 1.0;  // algorithmic bytes scale with the element size
         const void *Cfin_p = lowp ? d_Cfin32.get() : d_Cfin.get();
         const void *CWb_p = lowp ? d_CWb32.get() : d_CWb.get();
@@ -598,8 +606,8 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         if (mixed)
             for (auto &c : chunks) {
                 c.wb.use_tiles(cur, stream);
-                c.xb.use_tiles(cur, stream);
-                c.ta.use_tiles(cur, stream);
+                c.xb.use_tiles(small, stream);
+                c.ta.use_tiles(small, stream);
                 for (auto &f : c.fin) f.use_tiles(cur, stream);
             }
         for (int s = 0; s < nspin; ++s) {
@@ -638,13 +646,13 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                     void *Tb = (ci % 2) ? d_T2.get() : d_T.get();
                     // the contraction two chunks back has finished reading this buffer
                     if (ci >= 2) HIP_CHECK(hipStreamWaitEvent(s2, ev_c[ci % 2], 0));
-                    prof.begin(rn("chi0_TA"), s2);
-                    gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), Tb, 1.0, 0.0, s2);
-                    prof.end(rn("chi0_TA"), s2, c.ta.flops, c.ta_bytes * bsc);
+                    prof.begin(rs("chi0_TA"), s2);
+                    gemm_run(small, c.ta, d_Gpm.get(), d_W.get(), Tb, 1.0, 0.0, s2);
+                    prof.end(rs("chi0_TA"), s2, c.ta.flops, c.ta_bytes * bsc);
                     HIP_CHECK(hipEventRecord(ev_ta[ci % 2], s2));
-                    prof.begin(rn("chi0_X"), stream);
-                    gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
-                    prof.end(rn("chi0_X"), stream, c.xb.flops);
+                    prof.begin(rs("chi0_X"), stream);
+                    gemm_run(small, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                    prof.end(rs("chi0_X"), stream, c.xb.flops);
                     HIP_CHECK(hipStreamWaitEvent(stream, ev_ta[ci % 2], 0));
                     for (int k = 0; k < 4; ++k) {
                         if (c.fin[k].total_tiles == 0) continue;
@@ -668,16 +676,16 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
                 if (tune_now && c.ta.cfg_by_prec[(int)cur] < 0) {
                     // TA overwrites T; the contraction trials accumulate into chi0s before the X
                     // term below resets it (beta = 0) for this tau
-                    c.ta.tune(cur, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                    gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    c.ta.tune(small, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                    gemm_run(small, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
                     for (auto &f : c.fin) f.tune(cur, Cfin_p, d_T.get(), chi_base, 1.0, 1.0, stream);
                 }
-                prof.begin(rn("chi0_X"), stream);
-                gemm_run(cur, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
-                prof.end(rn("chi0_X"), stream, c.xb.flops);
-                prof.begin(rn("chi0_TA"), stream);
-                gemm_run(cur, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
-                prof.end(rn("chi0_TA"), stream, c.ta.flops, c.ta_bytes * bsc);
+                prof.begin(rs("chi0_X"), stream);
+                gemm_run(small, c.xb, d_W.get(), d_W.get(), chi_base, 1.0, s == 0 ? 0.0 : 1.0, stream);
+                prof.end(rs("chi0_X"), stream, c.xb.flops);
+                prof.begin(rs("chi0_TA"), stream);
+                gemm_run(small, c.ta, d_Gpm.get(), d_W.get(), d_T.get(), 1.0, 0.0, stream);
+                prof.end(rs("chi0_TA"), stream, c.ta.flops, c.ta_bytes * bsc);
                 // one timing region per launch (per block-shape class)
                 for (int k = 0; k < 4; ++k) {
                     if (c.fin[k].total_tiles == 0) continue;

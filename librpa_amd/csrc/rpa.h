@@ -70,6 +70,7 @@ struct RpaEngine {
     bool mixed = false;
     double mixed_amax = 1e3;
     int n_lowp_tau = 0;  // tau points of the last build that ran in FP32 (mixed mode)
+    Prec mixed_low = Prec::F32;  // precision of those tau points
     Prec cur = Prec::F64;  // precision of the tau point being built
     DevBuf d_Cfin32, d_CWb32, d_chi32;
     DevBuf d_Cfin, d_CWb, d_ZZ, d_Zs, d_Mst, d_Gpm, d_Ghat, d_W, d_T, d_T2, d_chi0s;

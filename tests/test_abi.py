@@ -87,12 +87,13 @@ int main(void) {
 
 
 def test_precision_selection_values(lib):
-    # chi0 precision 0..3 (3 = mixed tau precision), Ec precision 0..2; anything else is an error
+    # chi0 precision 0..4 (3, 4 = mixed tau precision with FP32 / BF16x3 at the low-weight
+    # points), Ec precision 0..2; anything else is an error
     import librpa_amd as L
     h = L.Handler()
-    for chi0 in (L.FP64, L.FP32, L.BF16X3, L.MIXED):
+    for chi0 in (L.FP64, L.FP32, L.BF16X3, L.MIXED, L.MIXED_BF16):
         for ec in (L.FP64, L.FP32, L.BF16X3):
             h.set_precision(chi0, ec)
-    for chi0, ec in ((4, 0), (-1, 0), (0, 3), (3, 3)):
+    for chi0, ec in ((5, 0), (-1, 0), (0, 3), (3, 3)):
         with pytest.raises(L.LibrpaError, match="unknown precision"):
             h.set_precision(chi0, ec)

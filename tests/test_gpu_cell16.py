@@ -114,7 +114,7 @@ def test_cell16_small_chunks_match_oracle(system, golden, fp64, monkeypatch, pre
         assert abs(ec - d["ec_ha"]) / system.natom < 1e-5
     grid = h.get_tfgrids(o)
     for it in (0, 12, 17):
-        low = prec == L.MIXED and np.abs(grid["cos_t2f"][:, it]).max() <= h.get_stats()["mixed_amax"]
+        low = prec in (L.MIXED, L.MIXED_BF16) and np.abs(grid["cos_t2f"][:, it]).max() <= h.get_stats()["mixed_amax"]
         ref = fp64["blocks"][it]
         got = _blocks(h, system, it)
         err = max(np.abs(got[k] - ref[k]).max() for k in ref) / max(np.abs(b).max() for b in ref.values())
@@ -141,13 +141,14 @@ def test_cell16_reduced_precision_report(system, golden, fp64, prec, tol_chi):
 
 
 @pytest.mark.gpu
-def test_cell16_mixed_precision(system, golden, fp64):
-    # MIXED: FP64 at the tau points whose cosine-transform column has an entry above 1e3, FP32
-    # at the others.  The FP64 points must reproduce the FP64 run, the FP32 ones carry FP32
-    # arithmetic errors, and Ec holds the 1e-5 Ha/atom bar.
+@pytest.mark.parametrize("mode", [L.MIXED, L.MIXED_BF16])
+def test_cell16_mixed_precision(system, golden, fp64, mode):
+    # MIXED (MIXED_BF16): FP64 at the tau points whose cosine-transform column has an entry above
+    # 1e3, FP32 (BF16x3) at the others.  The FP64 points must reproduce the FP64 run, the others
+    # carry their arithmetic's errors, and Ec holds the 1e-5 Ha/atom bar.
     d, _ = golden
-    _say("cell16 mixed precision")
-    h, o, ec, _ = _run(system, L.MIXED)
+    _say(f"cell16 mixed precision, mode {mode}")
+    h, o, ec, _ = _run(system, mode)
     grid = h.get_tfgrids(o)
     amax = np.abs(grid["cos_t2f"]).max(axis=0)
     st = h.get_stats()
