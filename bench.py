@@ -73,10 +73,14 @@ def screening_report(L, h, o, s, ec64, thr=1e-8):
     return out
 
 
+EC_PREC = None  # --ec-prec: the Ec transform precision of the mixed modes (default FP32)
+
+
 def run_steps(L, h, o, nk, prec, steps, warmup):
     # the mixed modes are chi0 settings; with them the Ec transforms (chi0 L, L^T X) run in FP32, which
     # the tau->omega transform does not amplify (the log-determinant stays FP64)
-    h.set_precision(prec, L.FP32 if prec in (L.MIXED, L.MIXED_BF16) else prec)
+    mixed_ec = L.FP32 if EC_PREC is None else EC_PREC
+    h.set_precision(prec, mixed_ec if prec in (L.MIXED, L.MIXED_BF16) else prec)
     ec = None
     for _ in range(warmup):
         ec, _ = h.get_rpa_correlation_energy(o, nk)
@@ -380,11 +384,17 @@ def main():
     ap.add_argument("--task", default="rpa", choices=["rpa", "g0w0", "screen"],
                     help="rpa: the chi0 + Ec metric (default); g0w0: W and Sigma_c timing and QP precision "
                          "on --workload; screen: block-norm screening on the 256-atom slab")
+    ap.add_argument("--ec-prec", default="auto", choices=["auto", "fp64", "fp32", "bf16x3"],
+                    help="Ec transform precision of the mixed modes (auto: fp32)")
     ap.add_argument("--screen-modes", default="unscreened,learned,bound",
                     help="--task screen: the slab runs, in order (comma-separated: unscreened, learned, bound)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    global EC_PREC
+    if args.ec_prec != "auto":
+        import librpa_amd as _L
+        EC_PREC = _L.PREC[args.ec_prec]
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
               "(no inter-GPU sharding in this build)", file=sys.stderr)

@@ -705,7 +705,8 @@ void RpaEngine::build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g) {
         if (lowp)
             launch_convert<float, double>(d_chi32.as<float>(), d_chi0s.as<double>() + (host_staged ? 0 : (size_t)it * E_tot),
                                           E_tot, stream);
-        // a host-staged build keeps its slice in pinned memory (the next tau waits on the copy)
+        // a host-staged build keeps its slice in pageable host memory (stream order makes the next
+        // tau's kernels wait for the copy)
         if (host_staged)
             HIP_CHECK(hipMemcpyAsync(h_chi0s[it].data(), d_chi0s.get(), ts * (size_t)E_tot, hipMemcpyDeviceToHost,
                                      stream));

@@ -154,7 +154,7 @@ def test_cell16_mixed_precision(system, golden, fp64, mode):
     st = h.get_stats()
     low = [t for t in range(system.nfreq) if amax[t] <= st["mixed_amax"]]
     assert len(low) == st["n_fp32_tau"]
-    _say(f"  FP32 tau points {low}; Ec {ec!r} oracle {d['ec_ha']!r}; dEc/atom {(ec - d['ec_ha']) / system.natom:.3e} Ha")
+    _say(f"  low-precision tau points {low}; Ec {ec!r} oracle {d['ec_ha']!r}; dEc/atom {(ec - d['ec_ha']) / system.natom:.3e} Ha")
     assert 0 < len(low) < system.nfreq
     for it in sorted(set(CHECK_TAU) | set(REDUCED_TAU)):
         ref = fp64["blocks"][it]

@@ -147,7 +147,7 @@ def test_mixed_precision_against_fp64(case, mode, tol_low):
                 e32 = max(e32, e)
             else:
                 e64 = max(e64, e)
-    _say(f"  FP32 tau {sorted(low)}: rel err fp64 points {e64:.1e}, fp32 points {e32:.1e}; "
+    _say(f"  low-precision tau {sorted(low)}: rel err fp64 points {e64:.1e}, low points {e32:.1e}; "
          f"Ec {ec!r} fp64 {ec64!r} dEc/atom {(ec - ec64) / s.natom:.3e}")
     assert e64 <= 1e-12 and e32 < tol_low
     assert abs(ec - ec64) / s.natom < 1e-5
