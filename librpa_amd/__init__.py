@@ -355,6 +355,11 @@ class Handler:
         m = np.ascontiguousarray(mapping, dtype=np.int32)
         self._call("librpa_set_ibz_mapping", len(m), m.ctypes.data_as(_ip))
 
+    def set_dielect_func_imagfreq(self, omegas, df):
+        om, po = _d(omegas)
+        v, pv = _d(df)
+        self._call("librpa_set_dielect_func_imagfreq", len(om), po, pv)
+
     def set_lri_coeff(self, routing, I, J, nbasis_i, nbasis_j, naux_mu, R, cs):
         r = (C.c_int * 3)(*[int(x) for x in R])
         c, pc = _d(cs)

@@ -61,7 +61,7 @@ def build(verbose=True):
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(hipcc, s), srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lrocsolver", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

@@ -7,7 +7,14 @@
 // the augmented matrix [[M, L^H], [L, v]] after eliminating its first block column with
 // the signed blocked Cholesky; complex q use the real embedding [[Re, -Im], [Im, Re]].
 // W(R, tau) = Re sum_w gamma_f2t(tau, w) (1/N_k) sum_q e^{-i 2 pi q.R} Wc(q, w)
-// (CT_FT_Wc_freq_q, epsilon.cpp:2490-2666) is accumulated per q on the device.
+// (CT_FT_Wc_freq_q, epsilon.cpp:2490-2666) is accumulated per q on the device.  With an IBZ
+// mapping only the irreducible q are formed; the others must be their time-reversal partners
+// (FT_Wc_freq_q, epsilon.cpp:2430-2452, takes W(-q) = conj W(q)), whose real parts of
+// e^{-i 2 pi q.R} W(q) are equal, so each irreducible q is weighted by its star size.
+// A Coulomb matrix with a negative pivot, a positive sqrt_coulomb_threshold or a head
+// replacement at q = 0 takes the square roots from an eigen-decomposition instead
+// (power_hemat, src/math/matrix_m.h:916-970, eigenvalues below the threshold dropped); then
+// W = S_c eps^{-1} S_c - S_c S_c, the Schur complement of [[eps, S_c], [S_c, S_c S_c]].
 //
 // Self-energy (RI::GW::cal_Sigmas, thirdparty/LibRI/include/RI/physics/GW.hpp:102-122,
 // called from G0W0::build_spacetime, src/core/gw.cpp:217-875), per tau, spin and sign:
@@ -22,6 +29,9 @@
 // atomic accumulation and are folded back to the BvK cell afterwards.  Sigma(R, i w) =
 // sum_tau 0.5 (S+ + S-) gamma_c + i 0.5 (S+ - S-) gamma_s (gw.cpp:681-797).
 #include "gw.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -446,6 +456,65 @@ void ns_sqrt(double *A, int n, hipStream_t st) {
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// Filtered square root of the symmetric n x n matrix A (in place): A <- sum_{lam_j >= thr, lam_j > 0}
+// lam_j^{1/2} u_j u_j^T from its eigen-decomposition (power_hemat, src/math/matrix_m.h:916-970;
+// the real embedding of a Hermitian matrix has every eigenvalue twice and the embedded root).
+// The eigen-decomposition is rocSOLVER's syevd, once per q; the root is one GEMM B^T B with the
+// eigenvectors scaled by lam^{1/4}.  top (optional, length n / halves): the eigenvector of the
+// largest eigenvalue, taken from one diagonal half of an embedded matrix when halves = 2.
+// Returns the largest and second largest eigenvalue.
+std::pair<double, double> eig_sqrt(double *A, int n, double thr, double *top, int halves, hipStream_t st) {
+    static rocblas_handle handle = nullptr;
+    if (!handle && rocblas_create_handle(&handle) != rocblas_status_success)
+        LRA_THROW("rocblas_create_handle failed");
+    if (rocblas_set_stream(handle, st) != rocblas_status_success) LRA_THROW("rocblas_set_stream failed");
+    const size_t nn = (size_t)n * n;
+    DevBuf dU(nn * 8), dW((size_t)n * 8), dE((size_t)n * 8), dinfo(8), df((size_t)n * 8);
+    HIP_CHECK(hipMemcpyAsync(dU.get(), A, nn * 8, hipMemcpyDeviceToDevice, st));
+    // column-major eigenvectors of a symmetric matrix: row j of the row-major view is u_j
+    if (rocsolver_dsyevd(handle, rocblas_evect_original, rocblas_fill_lower, n, dU.as<double>(), n, dW.as<double>(),
+                         dE.as<double>(), reinterpret_cast<rocblas_int *>(dinfo.get())) != rocblas_status_success)
+        LRA_THROW("rocsolver_dsyevd failed");
+    std::vector<double> lam(n);
+    int info = 0;
+    HIP_CHECK(hipMemcpyAsync(lam.data(), dW.get(), (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&info, dinfo.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (info != 0) LRA_THROW("eigen-decomposition of the Coulomb matrix did not converge");
+    if (top) {
+        std::vector<double> u(n);
+        HIP_CHECK(hipMemcpyAsync(u.data(), dU.as<double>() + (size_t)(n - 1) * n, (size_t)n * 8, hipMemcpyDeviceToHost,
+                                 st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        const int h = n / halves;
+        int best = 0;
+        double bn = -1;
+        for (int c = 0; c < halves; ++c) {
+            double s2 = 0;
+            for (int i = 0; i < h; ++i) s2 += u[(size_t)c * h + i] * u[(size_t)c * h + i];
+            if (s2 > bn) { bn = s2; best = c; }
+        }
+        std::vector<double> t(u.begin() + (size_t)best * h, u.begin() + (size_t)(best + 1) * h);
+        const double nr = std::sqrt(bn);
+        for (double &x : t) x /= nr;
+        HIP_CHECK(hipMemcpyAsync(top, t.data(), (size_t)h * 8, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+    std::vector<double> f(n);
+    for (int j = 0; j < n; ++j) f[j] = (lam[j] >= thr && lam[j] > 0) ? std::pow(lam[j], 0.25) : 0.0;
+    HIP_CHECK(hipMemcpyAsync(df.get(), f.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
+    launch_scale_rows(dU.as<double>(), n, df.as<double>(), st);
+    GemmBatch b;
+    b.trans_a = true;
+    b.add(make_prob(n, n, n, 0, n, 0, n, 0, n));
+    b.finalize(T64x64, st);
+    gemm_run(Prec::F64, b, dU.get(), dU.get(), A, 1.0, 0.0, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    // largest distinct eigenvalues (an embedded matrix repeats each one)
+    const double l0 = lam[n - 1], l1 = lam[std::max(0, n - 1 - halves)];
+    return {l0, l1};
+}
+
 // The upper atom-pair blocks (I <= J) of a Coulomb input for every k, staged on the device for
 // launch_v_scatter, with the largest |V| and |Im V| of each k.
 struct CoulombStage {
@@ -502,9 +571,25 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     // no truncated Coulomb: v_c^{1/2} = 0 and W = 0 (epsilon.cpp:1862-1879)
     w_zero = ds.vq_cut.empty();
     if (w_zero) { HIP_CHECK(hipStreamSynchronize(st)); t_w = now_s() - t0; return; }
-    if (!ds.ibz_map.empty())
-        for (int k = 0; k < nk; ++k)
-            if (ds.ibz_map[k] != k) LRA_THROW("G0W0 needs the full k grid (no IBZ reduction) in this build");
+    // irreducible q and their star sizes; a reducible q must be the time-reversal partner of its
+    // irreducible q (the assumption of FT_Wc_freq_q, epsilon.cpp:2430-2452)
+    std::vector<int> mult(nk, 0);
+    for (int k = 0; k < nk; ++k) {
+        const int r = ds.ibz_map.empty() ? k : ds.ibz_map[k];
+        if (r < 0 || r >= nk || (!ds.ibz_map.empty() && ds.ibz_map[r] != r))
+            LRA_THROW("IBZ mapping: k point " + std::to_string(k) + " maps to a point that is not irreducible");
+        if (r != k) {
+            bool partner = true;
+            for (int d = 0; d < 3; ++d) {
+                const double sum = ds.kfrac[(size_t)k * 3 + d] + ds.kfrac[(size_t)r * 3 + d];
+                partner = partner && std::fabs(sum - std::round(sum)) < 1e-8;
+            }
+            if (!partner)
+                LRA_THROW("G0W0 with an IBZ mapping needs every reducible q to be -q of its irreducible q (k point " +
+                          std::to_string(k) + ")");
+        }
+        mult[r]++;
+    }
     // v_c = v lets W = L M^{-1} L^T - v use the Cholesky factor of v; otherwise the square roots
     // of v and v_c are formed (Newton-Schulz) and W = v_c^{1/2} eps^{-1} v_c^{1/2} - v_c
     bool same_cut = ds.vq_cut.size() == ds.vq.size();
@@ -527,31 +612,52 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     const double budget = std::min(160e9, 0.7 * (double)free_b) - 8.0 * 10 * nn;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
     DevBuf dV0(nn * 8), dL(nn * 8), dD(nn * 8 * WCH), dX(nn * 8 * WCH), dA(NN * 8 * WCH), dacc(sizeof(double) * (WCH + 8));
-    // pivot signs of v = L S L^T (v_c = v): W needs v^{1/2}, which a v with a negative pivot lacks
-    // (power_hemat, matrix_m.h:916-970, would take the square root of a negative eigenvalue)
-    DevBuf dSg((size_t)n2 * 8);
+    // pivot signs of v = L S L^T: a negative pivot means v^{1/2} must come from the eigen-decomposition
+    // with the negative eigenvalues dropped (power_hemat, matrix_m.h:916-970)
+    DevBuf dSg((size_t)n2 * 8), dProbe, dProbeFlag(16);
     std::vector<double> hsg;
     int *dflag = reinterpret_cast<int *>(dacc.as<double>() + WCH + 4);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), st));
     CholWork cw;
     DevBuf dSc;
+    auto negative_pivot = [&](const double *M) {
+        dProbe.reserve(nn * 8);
+        HIP_CHECK(hipMemcpyAsync(dProbe.get(), M, nn * 8, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemsetAsync(dProbeFlag.get(), 0, 16, st));
+        cholesky_logdet(dProbe.as<double>(), n2, n2, dProbeFlag.as<double>(), reinterpret_cast<int *>(dProbeFlag.as<double>() + 1),
+                        true, cw, st, dSg.as<double>());
+        hsg.resize(n2);
+        HIP_CHECK(hipMemcpyAsync(hsg.data(), dSg.get(), (size_t)n2 * 8, hipMemcpyDeviceToHost, st));
+        int zf = 0;
+        HIP_CHECK(hipMemcpyAsync(&zf, dProbeFlag.as<double>() + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        bool neg = zf != 0;  // a zero pivot also needs the filtered root
+        for (double v : hsg) neg = neg || v < 0;
+        return neg;
+    };
+    // head replacement at q = 0: eps_mac per frequency and the leading eigenvector of Re v(0)
+    DevBuf dHead, dU0((size_t)n * 8), dQ(sizeof(double) * (WCH + 1));
+    if (!head.empty()) {
+        if ((int)head.size() != nw) LRA_THROW("dielectric function: one value per frequency of the grid is needed");
+        dHead = upload(head, st);
+    }
     CoulombStage vst, vcst;
     stage_coulomb(R0, ds.vq, nk, vst, st);
     if (!same_cut) stage_coulomb(R0, ds.vq_cut, nk, vcst, st);
-    std::map<int, std::pair<GemmBatch, GemmBatch>> batches;
-    auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
-        auto it = batches.find(cnt);
+    std::map<std::pair<int, bool>, std::pair<GemmBatch, GemmBatch>> batches;
+    auto get_batches = [&](int cnt, bool tri) -> std::pair<GemmBatch, GemmBatch> & {
+        auto it = batches.find({cnt, tri});
         if (it != batches.end()) return it->second;
-        auto &pr = batches[cnt];
+        auto &pr = batches[{cnt, tri}];
         GemmBatch &bx = pr.first, &bm = pr.second;
         bm.trans_a = true;
         bm.lower_only = true;
         for (int w = 0; w < cnt; ++w) {
             GemmProb px = make_prob(n2, n2, n2, (long long)w * nn, n2, 0, n2, (long long)w * nn, n2);
-            px.kmode = same_cut ? 1 : 0;  // triangular Cholesky factor, or the full square root
+            px.kmode = tri ? 1 : 0;  // triangular Cholesky factor, or the full square root
             bx.add(px);
             GemmProb pm = make_prob(n2, n2, n2, 0, n2, (long long)w * nn, n2, (long long)w * NN, N);
-            pm.kmode = same_cut ? 2 : 0;
+            pm.kmode = tri ? 2 : 0;
             bm.add(pm);
         }
         const int cfg = T64x64;
@@ -563,36 +669,66 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     };
     const auto &cells = R0.cells;
     for (int iq = 0; iq < nk; ++iq) {
+        if (mult[iq] == 0) continue;
         const double *kf = &ds.kfrac[(size_t)iq * 3];
         const bool gamma = std::fabs(kf[0]) < 1e-12 && std::fabs(kf[1]) < 1e-12 && std::fabs(kf[2]) < 1e-12;
         const double vmax = vst.vmax[iq], imax = vst.imax[iq];
+        const bool head_q = gamma && !head.empty();
+        bool eig_v = sqrt_threshold > 0 || head_q, eig_vc = sqrt_threshold > 0;
+        bool chol = false;  // v_c = v with v = L S L^T, S = I: the Cholesky identity
         // v^{1/2} from Re v at q = 0 (power_hemat_blacs_real, epsilon.cpp:2019-2024)
-        if (same_cut) {
-            // with v_c = v the Cholesky identity needs one matrix for both roots: real v(0)
-            if (gamma && imax > 1e-12 * std::max(vmax, 1e-300))
-                LRA_THROW("W at q = 0 with v_c = v needs a real Coulomb matrix (Im v(0) != 0)");
+        if (same_cut && gamma && imax > 1e-12 * std::max(vmax, 1e-300))
+            LRA_THROW("W at q = 0 with v_c = v needs a real Coulomb matrix (Im v(0) != 0)");
+        if (same_cut && !eig_v) {
             dense_coulomb(vst, iq, n, n2, embed, gamma, dV0.as<double>(), st);
             HIP_CHECK(hipMemcpyAsync(dL.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
             prof.begin("gw_w_chol_v", st);
-            cholesky_logdet(dL.as<double>(), n2, n2, dacc.as<double>() + WCH + 2, dflag + 1, true, cw, st,
-                            dSg.as<double>());
+            HIP_CHECK(hipMemsetAsync(dProbeFlag.get(), 0, 16, st));
+            cholesky_logdet(dL.as<double>(), n2, n2, dProbeFlag.as<double>(),
+                            reinterpret_cast<int *>(dProbeFlag.as<double>() + 1), true, cw, st, dSg.as<double>());
             hsg.resize(n2);
+            int zf = 0;
             HIP_CHECK(hipMemcpyAsync(hsg.data(), dSg.get(), (size_t)n2 * 8, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(&zf, dProbeFlag.as<double>() + 1, sizeof(int), hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipStreamSynchronize(st));
-            for (double v : hsg)
-                if (v < 0) LRA_THROW("W needs a positive-definite Coulomb matrix (negative pivot of v, no v^1/2)");
+            chol = zf == 0;  // a zero or negative pivot needs the filtered root
+            for (double v : hsg) chol = chol && v > 0;
+            eig_v = eig_vc = !chol;
             prof.end("gw_w_chol_v", st, (double)n2 * n2 * n2 / 3.0);
-        } else {
-            // dL <- v^{1/2}; dV0 <- v_c, dSc <- v_c^{1/2} (complex v_c kept, as the reference does)
-            if (!embed && vcst.imax[iq] > 1e-12 * std::max(vcst.vmax[iq], 1e-300))
+        }
+        if (!chol) {
+            // dL <- S = v^{1/2}; dSc <- S_c = v_c^{1/2} (complex v_c kept, as the reference does);
+            // dV0 <- v_c, or S_c S_c when a root is filtered
+            if (!same_cut && !embed && vcst.imax[iq] > 1e-12 * std::max(vcst.vmax[iq], 1e-300))
                 LRA_THROW("a complex truncated Coulomb matrix at Gamma-only sampling is outside this build");
             prof.begin("gw_w_sqrt_v", st);
             dense_coulomb(vst, iq, n, n2, embed, gamma, dL.as<double>(), st);
-            ns_sqrt(dL.as<double>(), n2, st);
-            dense_coulomb(vcst, iq, n, n2, embed, false, dV0.as<double>(), st);
+            if (!eig_v) eig_v = negative_pivot(dL.as<double>());
+            if (eig_v) {
+                const auto top2 = eig_sqrt(dL.as<double>(), n2, sqrt_threshold, head_q ? dU0.as<double>() : nullptr,
+                                           embed ? 2 : 1, st);
+                if (head_q && !(top2.first - top2.second > 1e-8 * std::fabs(top2.first)))
+                    LRA_THROW("head of the dielectric matrix: the largest eigenvalue of v(0) is degenerate");
+            } else {
+                ns_sqrt(dL.as<double>(), n2, st);
+            }
             dSc.reserve(nn * 8);
-            HIP_CHECK(hipMemcpyAsync(dSc.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
-            ns_sqrt(dSc.as<double>(), n2, st);
+            if (same_cut) {
+                eig_vc = eig_v;
+                HIP_CHECK(hipMemcpyAsync(dSc.get(), dL.get(), nn * 8, hipMemcpyDeviceToDevice, st));
+            } else {
+                dense_coulomb(vcst, iq, n, n2, embed, false, dV0.as<double>(), st);
+                HIP_CHECK(hipMemcpyAsync(dSc.get(), dV0.get(), nn * 8, hipMemcpyDeviceToDevice, st));
+                if (!eig_vc) eig_vc = negative_pivot(dSc.as<double>());
+                if (eig_vc) eig_sqrt(dSc.as<double>(), n2, sqrt_threshold, nullptr, 1, st);
+                else ns_sqrt(dSc.as<double>(), n2, st);
+            }
+            if (eig_v || eig_vc) {
+                GemmBatch sq;
+                sq.add(make_prob(n2, n2, n2, 0, n2, 0, n2, 0, n2));
+                sq.finalize(T64x64, st);
+                gemm_run(Prec::F64, sq, dSc.get(), dSc.get(), dV0.get(), 1.0, 0.0, st);
+            }
             prof.end("gw_w_sqrt_v", st);
         }
         std::vector<double> csr(nR), snr(nR);
@@ -608,17 +744,25 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
             prof.begin("gw_w_chi0q", st);
             R0.assemble_dense(g, kf, w0, cnt, dD.get(), true, (long long)nn, embed, st);
             prof.end("gw_w_chi0q", st);
-            auto &bb = get_batches(cnt);
+            auto &bb = get_batches(cnt, chol);
             prof.begin("gw_w_gemm", st);
             gemm_run(Prec::F64, bb.first, dD.get(), dL.get(), dX.get(), 1.0, 0.0, st);
             HIP_CHECK(hipMemsetAsync(dA.get(), 0, (size_t)cnt * NN * 8, st));
             for (int w = 0; w < cnt; ++w) {
                 double *A = dA.as<double>() + (size_t)w * NN;
                 launch_set_identity<double>(A, n2, N, st);
-                launch_copy_block(same_cut ? dL.as<double>() : dSc.as<double>(), n2, A + (size_t)n2 * N, N, n2, n2, st);
+                launch_copy_block(chol ? dL.as<double>() : dSc.as<double>(), n2, A + (size_t)n2 * N, N, n2, n2, st);
                 launch_copy_block(dV0.as<double>(), n2, A + (size_t)n2 * N + n2, N, n2, n2, st);
             }
             gemm_run(Prec::F64, bb.second, dL.get(), dX.get(), dA.get(), -1.0, 1.0, st);
+            if (head_q) {
+                // eps += (eps_mac - u^T eps u) u u^T, u the leading eigenvector of Re v(0): the
+                // head overwrite of epsilon.cpp:2106-2181 in the eigenbasis of v
+                HIP_CHECK(hipMemsetAsync(dQ.get(), 0, sizeof(double) * cnt, st));
+                launch_head_quad(dA.as<double>(), (long long)NN, N, n, cnt, dU0.as<double>(), dQ.as<double>(), st);
+                launch_head_update(dA.as<double>(), (long long)NN, N, n, embed ? n : 0, cnt, dU0.as<double>(),
+                                   dHead.as<double>() + w0, dQ.as<double>(), st);
+            }
             prof.end("gw_w_gemm", st, bb.first.flops + bb.second.flops);
             prof.begin("gw_w_schur", st);
             cholesky_logdet_batched(dA.as<double>(), (long long)NN, cnt, N, N, dacc.as<double>(), 1, dflag, false, cw,
@@ -630,7 +774,8 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
                 const int cw_n = std::min(8, cnt - a0);
                 std::vector<double> coef((size_t)cw_n * g.n);
                 for (int w = 0; w < cw_n; ++w)
-                    for (int t = 0; t < g.n; ++t) coef[(size_t)w * g.n + t] = g.cos_f2t[(size_t)t * g.n + w0 + a0 + w] / nk;
+                    for (int t = 0; t < g.n; ++t)
+                        coef[(size_t)w * g.n + t] = g.cos_f2t[(size_t)t * g.n + w0 + a0 + w] * mult[iq] / nk;
                 DevBuf dcoef = upload(coef, st);
                 launch_w_accum(dA.as<double>() + (size_t)a0 * NN, (long long)NN, cw_n, n, n2, N, embed,
                                dcoef.as<double>(), g.n, dcs.as<double>(), dsn.as<double>(), nR, d_Wrt.as<double>(), st);
@@ -642,9 +787,9 @@ void GwEngine::build_w(const Dataset &ds, const TFGrid &g) {
     int hflag[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(hflag, dflag, sizeof(hflag), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    if (hflag[1]) LRA_THROW("Coulomb matrix is singular or not finite (zero pivot in its factorisation)");
     if (hflag[0]) LRA_THROW("1 - v^1/2 chi0 v^1/2 is singular or not finite at some (q, omega)");
-    flops_w = nk * ((double)n2 * n2 * n2 / 3.0 + nw * (2.0 * n2 * (double)n2 * n2 / 2.0 + 2.0 * n2 * (double)n2 * n2 / 6.0 +
+    const double nq_w = (double)std::count_if(mult.begin(), mult.end(), [](int m) { return m > 0; });
+    flops_w = nq_w * ((double)n2 * n2 * n2 / 3.0 + nw * (2.0 * n2 * (double)n2 * n2 / 2.0 + 2.0 * n2 * (double)n2 * n2 / 6.0 +
                                                        (7.0 / 3.0) * n2 * (double)n2 * n2));
     t_w = now_s() - t0;
 }
@@ -972,11 +1117,21 @@ int qpe_solve(const PadeAC &p, double e_mf, double e_fermi, double vxc, double s
 // ------------------------------------------------------------------------------------------
 void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
     if (o.use_shrink_abfs == LIBRPA_SWITCH_ON) LRA_THROW("ABF shrinking is outside this build's path");
-    if (o.replace_w_head == LIBRPA_SWITCH_ON && !d.dielect_omega.empty())
-        LRA_THROW("head/wing dielectric corrections of W are outside this build's path");
-    if (o.sqrt_coulomb_threshold > 0)
-        LRA_THROW("a positive sqrt_coulomb_threshold (eigenvalue filtering of v) is outside this build's path");
+    if (o.sqrt_coulomb_threshold < 0) LRA_THROW("sqrt_coulomb_threshold must not be negative");
     lra_prepare_grid(d, o);
+    // head of the dielectric matrix from the macroscopic dielectric function
+    // (librpa_get_g0w0_sigc_kgrid path, src/api/compute_g0w0.cpp:102-118): option 0 takes the
+    // input values as they are, one per grid frequency (interpolate_dielec_func,
+    // src/core/dielecmodel.cpp:65-104)
+    std::vector<double> head;
+    if (o.replace_w_head == LIBRPA_SWITCH_ON && !d.dielect_omega.empty()) {
+        if (o.option_dielect_func != 0)
+            LRA_THROW("option_dielect_func " + std::to_string(o.option_dielect_func) +
+                      ": only 0 (the dielectric function given on the frequency grid) is in this build");
+        if ((int)d.dielect_val.size() != d.grid.n)
+            LRA_THROW("option_dielect_func 0 needs the dielectric function at the nfreq grid frequencies");
+        head = d.dielect_val;
+    }
     auto eng = d.rpa;
     if (!eng || eng->prec != d.chi0_prec || eng->mixed != d.chi0_mixed || eng->planned_ntau != d.grid.n) {
         d.rpa.reset();
@@ -989,6 +1144,8 @@ void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
     eng->release_work();  // chi0(R, tau) stays; the chi0 work buffers are rebuilt on demand
     auto gw = std::make_shared<GwEngine>();
     gw->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
+    gw->head = head;
+    gw->sqrt_threshold = o.sqrt_coulomb_threshold;
     gw->plan(d, eng);
     gw->build_w(d, d.grid);
     gw->build_sigma(d, d.grid);

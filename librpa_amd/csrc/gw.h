@@ -77,6 +77,10 @@ struct GwEngine {
     double flops_sigma = 0, flops_w = 0, t_w = 0, t_sigma = 0;
     long long max_ghat = 0, max_tdc = 0;
     bool verbose = false;  // progress lines on stderr
+    // eps_mac(i omega_w) per grid frequency: replaces the head of the dielectric matrix at q = 0
+    // (replace_w_head with option_dielect_func 0); empty = no replacement
+    std::vector<double> head;
+    double sqrt_threshold = 0.0;  // sqrt_coulomb_threshold: eigenvalues of v, v_c below it are dropped
     bool sigma_tuned = false;  // Sigma batches' tiles timed (LIBRPA_AMD_TUNE_SIGMA=1) on the first pass
 
     void plan(const Dataset &ds, std::shared_ptr<RpaEngine> r);

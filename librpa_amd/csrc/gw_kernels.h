@@ -38,6 +38,18 @@ void launch_sigma_ks_rows(const double *sre, const double *sim, int nw, int nR, 
 void launch_scale_add_identity(const double *A, double *B, int n, double alpha, double beta, hipStream_t s);
 // out[0] = max |A - B|, out[1] = max |B| (single workgroup, overwritten)
 void launch_absmax_diff(const double *A, const double *B, long long n, double *out, hipStream_t s);
+// A[i][:] *= f[i] (row-major n x n)
+void launch_scale_rows(double *A, int n, const double *f, hipStream_t s);
+
+// Head of the dielectric matrix at q = 0 (replace_w_head): for each of cnt matrices eps_w (lower
+// triangle valid, rows of length ld, stride apart), q[w] = u^T eps_w u over the n x n block at the
+// origin (q is accumulated, clear it first) ...
+void launch_head_quad(const double *A, long long stride, int ld, int n, int cnt, const double *u, double *q,
+                      hipStream_t s);
+// ... then eps_w += (eps_mac[w] - q[w]) u u^T on the lower triangle of that block and, when
+// off2 > 0, of the copy at (off2, off2) (the real embedding of a q = 0 matrix).
+void launch_head_update(double *A, long long stride, int ld, int n, int off2, int cnt, const double *u,
+                        const double *eps_mac, const double *q, hipStream_t s);
 // out[0] = sum A^2 (overwritten)
 void launch_sumsq(const double *A, long long n, double *out, hipStream_t s);
 

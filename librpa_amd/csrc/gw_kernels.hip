@@ -181,6 +181,59 @@ void launch_scale_add_identity(const double *A, double *B, int n, double alpha, 
     HIP_CHECK(hipGetLastError());
 }
 
+__global__ void scale_rows_kernel(double *__restrict__ A, int n, const double *__restrict__ f) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    A[idx] *= f[idx / n];
+}
+void launch_scale_rows(double *A, int n, const double *f, hipStream_t s) {
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, s, A, n, f);
+    HIP_CHECK(hipGetLastError());
+}
+
+// one workgroup per (row i, matrix w): u_i (eps_ii u_i + 2 sum_{j<i} eps_ij u_j), summed into q[w]
+__global__ __launch_bounds__(256) void head_quad_kernel(const double *__restrict__ A, long long stride, int ld,
+                                                        const double *__restrict__ u, double *q) {
+    __shared__ double red[256];
+    const int i = blockIdx.x, w = blockIdx.y;
+    const double *row = A + (long long)w * stride + (long long)i * ld;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < i; j += blockDim.x) acc += row[j] * u[j];
+    acc *= 2.0;
+    if (threadIdx.x == 0) acc += row[i] * u[i];
+    red[threadIdx.x] = acc * u[i];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(q + w, red[0]);
+}
+void launch_head_quad(const double *A, long long stride, int ld, int n, int cnt, const double *u, double *q,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(head_quad_kernel, dim3(n, cnt), dim3(256), 0, s, A, stride, ld, u, q);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void head_update_kernel(double *__restrict__ A, long long stride, int ld, int off2,
+                                                          const double *__restrict__ u,
+                                                          const double *__restrict__ eps_mac,
+                                                          const double *__restrict__ q) {
+    const int i = blockIdx.x, w = blockIdx.y;
+    const double d = (eps_mac[w] - q[w]) * u[i];
+    double *row = A + (long long)w * stride + (long long)i * ld;
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) {
+        const double add = d * u[j];
+        row[j] += add;
+        if (off2 > 0) row[(long long)off2 * ld + off2 + j] += add;
+    }
+}
+void launch_head_update(double *A, long long stride, int ld, int n, int off2, int cnt, const double *u,
+                        const double *eps_mac, const double *q, hipStream_t s) {
+    hipLaunchKernelGGL(head_update_kernel, dim3(n, cnt), dim3(256), 0, s, A, stride, ld, off2, u, eps_mac, q);
+    HIP_CHECK(hipGetLastError());
+}
+
 // single workgroup of 1024 threads: deterministic reductions without atomics
 __global__ __launch_bounds__(1024) void absmax_diff_kernel(const double *__restrict__ A, const double *__restrict__ B,
                                                            long long n, double *out) {

@@ -217,6 +217,13 @@ def read_dataset(dir_path, cs_threshold=1e-6) -> dict:
     return dict(band=band, stru=stru, bz=bz, nao=nao, nabf=nabf, wfc=wfc, cs=cs, vq=V)
 
 
+def read_dielec_func(path):
+    """``dielecfunc_out`` (read_dielec_func, driver/read_data.cpp:1211-1229): rows of omega, Re eps,
+    Im eps; returns (omega, Re eps)."""
+    t = np.array([float(x) for x in _tokens(path)]).reshape(-1, 3)
+    return t[:, 0].copy(), t[:, 1].copy()
+
+
 def read_vxc(path):
     """``vxc_out`` (read_vxc, driver/read_data.cpp:142-182): n_k, n_spin, n_states, then "Ha eV" per
     state, k outer, spin inner.  Returns [spin][k][state] in Ha."""

@@ -9,10 +9,15 @@ Restates LibRPA's LibRI-routing G0W0 chain:
   dropped (power_hemat, src/math/matrix_m.h:916-970).  At q = 0 the square root of v is
   taken from Re v (power_hemat_blacs_real, src/math/utils_matrix_m_mpi.h:886-990,
   called at epsilon.cpp:2019-2024); v_c keeps its full complex form.
+  With a macroscopic dielectric function (replace_w_head, option_dielect_func 0), the head
+  of the dielectric matrix at q = 0 is overwritten in the eigenbasis of Re v
+  (``eps_head_gamma``, epsilon.cpp:2106-2181).
 * ``w_r_tau``: ``CT_FT_Wc_freq_q`` (epsilon.cpp:2490-2666) with ``FT_Wc_freq_q``
   (:2323-2489): W(R, w) = (1/N_k) sum_q e^{-i 2 pi q.R} Wc(q, w) (N_k = 1: plain copy),
   W(R, tau) = sum_w gamma_f2t(tau, w) W(R, w); the real part of each atom-pair block
-  is what the real-tensor path hands to LibRI (src/core/gw.cpp:366-375).
+  is what the real-tensor path hands to LibRI (src/core/gw.cpp:366-375).  With an IBZ
+  mapping W is formed at the irreducible q only; a mapped q is taken as the time-reversal
+  partner -q with W(-q) = conj W(q) (epsilon.cpp:2430-2452).
 * ``sigma_labels``: ``RI::GW::cal_Sigmas`` (thirdparty/LibRI/include/RI/physics/GW.hpp:
   102-122), i.e. ``LRI::cal_loop3`` with labels a0b0_a1b1, a0b0_a1b2, a0b0_a2b1,
   a0b0_a2b2 (ri/LRI-cal_loop3.hpp:57-..): with C^I(K,Ra)(mu, i, k), W_IJ(R)(mu, nu),
@@ -65,29 +70,72 @@ def is_gamma(q):
     return bool(np.all(np.abs(np.asarray(q)) < 1e-12))
 
 
-def wc_q_omega(sys, grid, chi0q, vq=None, vq_cut=None, threshold=0.0):
-    """Wc[(iw, iq)] dense complex n_abf x n_abf."""
+def eps_head_gamma(V, chi, eps_mac, threshold=0.0):
+    """Dielectric matrix at q = 0 with its head replaced by the macroscopic dielectric function
+    (compute_Wc_freq_q_blacs, epsilon.cpp:2106-2181, option_dielect_func != 3): with the
+    eigen-decomposition Re v = U diag(lam) U^T in descending order and the columns with
+    lam >= threshold kept, M = (U lam^{1/2})^T chi (U lam^{1/2}); M[0, 0] is set to 1 - eps_mac,
+    and eps = 1 - U M U^T."""
+    w, U = np.linalg.eigh(V.real)
+    w, U = w[::-1], U[:, ::-1]
+    keep = int(np.count_nonzero(w >= threshold))
+    Un = U[:, :keep].astype(np.complex128)
+    S = Un * np.sqrt(w[:keep])[None, :]
+    M = S.conj().T @ chi @ S
+    M[0, 0] = 1.0 - eps_mac
+    return np.eye(V.shape[0]) - Un @ M @ Un.conj().T
+
+
+def ibz_partners(sys, ibz):
+    """Irreducible q of every q of the full grid and whether q is its time-reversal partner, as
+    FT_Wc_freq_q assumes (epsilon.cpp:2430-2452: a mapped q that is not the irreducible one is
+    taken to be -q, and W(-q) = conj W(q))."""
+    out = []
+    for iq, q in enumerate(sys.kfrac):
+        r = int(ibz[iq])
+        d = np.asarray(q) - np.asarray(sys.kfrac[r])
+        s = np.asarray(q) + np.asarray(sys.kfrac[r])
+        if np.all(np.abs(d - np.round(d)) < 1e-8):
+            out.append((r, False))
+        elif np.all(np.abs(s - np.round(s)) < 1e-8):
+            out.append((r, True))
+        else:
+            raise ValueError(f"q {iq} maps to {r}, which is neither q nor -q")
+    return out
+
+
+def wc_q_omega(sys, grid, chi0q, vq=None, vq_cut=None, threshold=0.0, head=None, ibz=None):
+    """Wc[(iw, iq)] dense complex n_abf x n_abf; with ``ibz`` (irreducible index of each q) only
+    the irreducible q are computed; ``head`` (eps_mac per omega) replaces the head of the
+    dielectric matrix at q = 0."""
     abf = Basis(sys.nabf)
     vq = sys.vq if vq is None else vq
     vq_cut = vq if vq_cut is None else vq_cut
     n = abf.total
     out = {}
     for iq, q in enumerate(sys.kfrac):
+        if ibz is not None and int(ibz[iq]) != iq:
+            continue
         V = vq[iq]
         sv = power_hemat(V.real.astype(np.complex128) if is_gamma(q) else V, 0.5, threshold)
         svc = power_hemat(vq_cut[iq], 0.5, threshold)
         for iw in range(len(grid['omega'])):
             chi = full_matrix(chi0q, abf, sys.natom, lambda I, J: chi0q[(iw, iq, I, J)])
-            eps = np.eye(n) - sv @ chi @ sv
+            if head is not None and is_gamma(q):
+                eps = eps_head_gamma(V, chi, head[iw], threshold)
+            else:
+                eps = np.eye(n) - sv @ chi @ sv
             out[(iw, iq)] = svc @ (np.linalg.inv(eps) - np.eye(n)) @ svc
     return out
 
 
-def w_r_tau(sys, grid, wq):
-    """W[(it, R)] real dense n_abf x n_abf, R on the BvK grid."""
+def w_r_tau(sys, grid, wq, ibz=None):
+    """W[(it, R)] real dense n_abf x n_abf, R on the BvK grid; with ``ibz`` the q outside the
+    irreducible set take conj W of their time-reversal partner."""
     nk = sys.nk
     Rs = r_grid(sys.period)
     nw = len(grid['omega'])
+    part = ibz_partners(sys, ibz) if ibz is not None else [(iq, False) for iq in range(nk)]
     out = {}
     for R in Rs:
         wr = []
@@ -97,7 +145,9 @@ def w_r_tau(sys, grid, wq):
             else:
                 acc = 0
                 for iq, q in enumerate(sys.kfrac):
-                    acc = acc + np.exp(-2j * np.pi * float(np.dot(q, R))) * wq[(iw, iq)]
+                    r, flip = part[iq]
+                    wmat = wq[(iw, r)].conj() if flip else wq[(iw, r)]
+                    acc = acc + np.exp(-2j * np.pi * float(np.dot(q, R))) * wmat
                 wr.append(acc / nk)
         for it in range(len(grid['tau'])):
             acc = 0

@@ -162,3 +162,35 @@ def test_ks_nearest_image_phase_on_grid_k():
     b = gw.sigma_ks_diag(s, sig, [0, 1], [0, 2], frac=None)
     for k in a:
         assert np.abs(a[k] - b[k]).max() < 1e-12
+
+
+def test_head_replacement_and_root_filter_change_w():
+    # the oracle's head replacement (eps_head_gamma) and eigenvalue filter (power_hemat with a
+    # threshold) are not no-ops: each moves W of a Gamma-only system by more than 1e-3 of its
+    # scale; and the head form equals the rank-1 update eps + (eps_mac - u^T eps u) u u^T with u
+    # the leading eigenvector of Re v (the library's form)
+    from librpa_amd import synth
+    from oracle import tfgrids
+    s = synth.tiny(natom=3, period=(1, 1, 1), nao=(3, 4), nabf=(5, 6), nocc=(1, 2), a=9.0, seed=5, nfreq=6,
+                   real_gamma_v=True)
+    emin, emax = tfgrids.e_min_max(s.eps, s.occ, s.nspin, s.nk)
+    g = tfgrids.minimax(s.nfreq, emin, emax)
+    chi0q = rpa.chi0_q_omega(s, g, rpa.chi0s_all_tau(s, g))
+    eps_mac = 1.0 + 6.0 / (1.0 + (g["omega"] / 0.5) ** 2)
+    base = gw.wc_q_omega(s, g, chi0q)
+    head = gw.wc_q_omega(s, g, chi0q, head=eps_mac)
+    thr = float(np.quantile(np.linalg.eigvalsh(s.vq[0]), 0.3))
+    filt = gw.wc_q_omega(s, g, chi0q, threshold=thr)
+    scale = max(np.abs(v).max() for v in base.values())
+    assert max(np.abs(head[k] - base[k]).max() for k in base) > 1e-3 * scale
+    assert max(np.abs(filt[k] - base[k]).max() for k in base) > 1e-3 * scale
+    abf = rpa.Basis(s.nabf)
+    V = s.vq[0]
+    w, U = np.linalg.eigh(V.real)
+    u = U[:, -1]
+    sv = gw.power_hemat(V.real.astype(np.complex128), 0.5)
+    for iw in range(s.nfreq):
+        chi = rpa.full_matrix(chi0q, abf, s.natom, lambda I, J: chi0q[(iw, 0, I, J)])
+        eps = np.eye(abf.total) - sv @ chi @ sv
+        rank1 = eps + (eps_mac[iw] - u @ eps @ u) * np.outer(u, u)
+        assert np.abs(rank1 - gw.eps_head_gamma(V, chi, eps_mac[iw])).max() < 1e-12 * np.abs(eps).max()
