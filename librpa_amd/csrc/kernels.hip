@@ -546,23 +546,25 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(double *A0, long long st
 }
 
 // Qs[r][c] = Q[r][c] * sgn[c] for matrix blockIdx.y of the batch; Q and Qs are m x nb regions of
-// pitch CHOL_PW inside that matrix's pair of wide panels
-constexpr int CHOL_PW = 8 * CHOL_NB;  // outer panel width: the trailing update runs with K = 512
+// the given pitch inside that matrix's pair of wide panels
+constexpr int CHOL_PW = 8 * CHOL_NB;   // middle panel: its update of the outer panel runs with K = 512
+constexpr int CHOL_PO = 4 * CHOL_PW;   // outer panel of large matrices: the trailing update has K = 2048
 
-__global__ void scale_cols_kernel(double *__restrict__ Q, double *__restrict__ Qs, long long pstride,
+__global__ void scale_cols_kernel(double *__restrict__ Q, double *__restrict__ Qs, long long pstride, int pitch,
                                   const double *__restrict__ work, int m, int nb) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)m * nb) return;
     const long long r = idx / nb, c = idx % nb;
     const double *sgn = work + (long long)blockIdx.y * (CHOL_NB * CHOL_NB + CHOL_NB) + CHOL_NB * CHOL_NB;
-    const long long o = (long long)blockIdx.y * pstride + r * CHOL_PW + c;
+    const long long o = (long long)blockIdx.y * pstride + r * pitch + c;
     Qs[o] = Q[o] * sgn[c];
 }
 
 struct CholPlan {
     std::vector<int> j0s, nbs;
     std::vector<GemmBatch> trsm, inpanel;  // per 64-column step
-    std::vector<GemmBatch> trail;          // per step: the K = CHOL_PW trailing update after an outer panel
+    std::vector<GemmBatch> mid;            // per step: the K = CHOL_PW update of the outer panel's other columns
+    std::vector<GemmBatch> trail;          // per step: the trailing update after an outer panel
 };
 
 static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan> &chol_plans() {
@@ -570,65 +572,85 @@ static std::map<std::tuple<int, int, int, long long, hipStream_t, int>, CholPlan
     return p;
 }
 
-// Right-looking blocked L S L^T factorisation, two levels: 64-column steps (diagonal block in
-// LDS, Q = A21 L11^-T by GEMM, signs applied) update only the rest of their 256-column outer
-// panel; after each outer panel one lower-triangular GEMM with K = CHOL_PW updates the trailing
-// matrix.  Q and Q S of the outer panel's rows are kept in two wide buffers (pitch CHOL_PW, indexed
-// by global row) per matrix.
+// Right-looking blocked L S L^T factorisation in three levels.  64-column steps (diagonal block
+// by potrf_diag_kernel, Q = A21 L11^-T by GEMM, signs applied) update only the rest of their
+// 512-column middle panel (K = 64); after each middle panel one GEMM with K = 512 updates the
+// remaining columns of its outer panel; after each outer panel one lower-triangular GEMM updates
+// the trailing matrix with K = the outer panel width (2048 for n >= 4096, else 512, when the
+// middle level is the outer one).  Few, deep trailing updates read and write the trailing matrix
+// four times less often than 512-wide ones.  Q and Q S of the outer panel's columns are kept in
+// two wide buffers per matrix (pitch = outer panel width, indexed by global row).
 void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, int lda, double *logdet,
                              long long ld_stride, int *flag, bool keep_L, CholWork &w, hipStream_t s, int ncols,
                              double *signs, long long sstride) {
     const int nf = (ncols > 0 && ncols < n) ? ncols : n;
+    const int PO = n >= 4096 ? CHOL_PO : CHOL_PW;
     const long long wstride = CHOL_NB * CHOL_NB + CHOL_NB;
-    const long long pstride = 2LL * n * CHOL_PW;  // Q then Q S, each n x CHOL_PW
+    const long long pstride = 2LL * n * PO;  // Q then Q S, each n x PO
     w.linv.reserve(sizeof(double) * wstride * nbatch);
     w.panel.reserve(sizeof(double) * pstride * nbatch);
-    const long long qs_off = (long long)n * CHOL_PW;
+    const long long qs_off = (long long)n * PO;
     auto key = std::make_tuple(n, lda, nbatch, strideA, s, nf);
     auto &plans = chol_plans();
     auto it = plans.find(key);
     if (it == plans.end()) {
         CholPlan cp;
-        for (int p0 = 0; p0 < nf; p0 += CHOL_PW) {
-            const int pend = std::min(p0 + CHOL_PW, nf);
-            for (int j0 = p0; j0 < pend; j0 += CHOL_NB) {
-                const int nb = std::min(CHOL_NB, pend - j0);
-                const int m = n - j0 - nb;
-                const int wcols = pend - j0 - nb;  // panel columns right of this step
-                cp.j0s.push_back(j0);
-                cp.nbs.push_back(nb);
-                GemmBatch tr, ip, tl;
-                if (m > 0) {
-                    tr.trans_b = true;
-                    ip.trans_b = true;
-                    for (int b = 0; b < nbatch; ++b) {
-                        const long long wq = pstride * b + (long long)(j0 + nb) * CHOL_PW + (j0 - p0);
-                        // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)) into the wide buffer
-                        tr.add(make_prob(m, nb, nb, strideA * b + (long long)(j0 + nb) * lda + j0, lda, wstride * b,
-                                         CHOL_NB, wq, CHOL_PW));
-                        // A[j0+nb:, j0+nb:pend] -= (Q S) Q[0:wcols]^T
-                        if (wcols > 0)
-                            ip.add(make_prob(m, wcols, nb, wq + qs_off, CHOL_PW, wq, CHOL_PW,
-                                             strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
+        for (int o0 = 0; o0 < nf; o0 += PO) {
+            const int oend = std::min(o0 + PO, nf);
+            for (int p0 = o0; p0 < oend; p0 += CHOL_PW) {
+                const int pend = std::min(p0 + CHOL_PW, oend);
+                for (int j0 = p0; j0 < pend; j0 += CHOL_NB) {
+                    const int nb = std::min(CHOL_NB, pend - j0);
+                    const int m = n - j0 - nb;
+                    const int wcols = pend - j0 - nb;  // middle panel columns right of this step
+                    cp.j0s.push_back(j0);
+                    cp.nbs.push_back(nb);
+                    GemmBatch tr, ip, md, tl;
+                    if (m > 0) {
+                        tr.trans_b = true;
+                        ip.trans_b = true;
+                        for (int b = 0; b < nbatch; ++b) {
+                            const long long wq = pstride * b + (long long)(j0 + nb) * PO + (j0 - o0);
+                            // Q = A21 * Linv^T  (A21: m x nb at (j0+nb, j0)) into the wide buffer
+                            tr.add(make_prob(m, nb, nb, strideA * b + (long long)(j0 + nb) * lda + j0, lda,
+                                             wstride * b, CHOL_NB, wq, PO));
+                            // A[j0+nb:, j0+nb:pend] -= (Q S) Q[0:wcols]^T
+                            if (wcols > 0)
+                                ip.add(make_prob(m, wcols, nb, wq + qs_off, PO, wq, PO,
+                                                 strideA * b + (long long)(j0 + nb) * lda + (j0 + nb), lda));
+                        }
+                        tr.finalize(T64x64, s);
+                        if (wcols > 0) ip.finalize(T64x64, s);
                     }
-                    tr.finalize(T64x64, s);
-                    if (wcols > 0) ip.finalize(T64x64, s);
-                }
-                const int mt = n - pend;
-                if (j0 + nb == pend && mt > 0) {
-                    // trailing update of the outer panel: A22 -= (Q S) Q^T over its pend - p0 columns
-                    tl.trans_b = true;
-                    tl.lower_only = true;
-                    for (int b = 0; b < nbatch; ++b) {
-                        const long long wq = pstride * b + (long long)pend * CHOL_PW;
-                        tl.add(make_prob(mt, mt, pend - p0, wq + qs_off, CHOL_PW, wq, CHOL_PW,
-                                         strideA * b + (long long)pend * lda + pend, lda));
+                    if (j0 + nb == pend && pend < oend) {
+                        // the middle panel's update of the outer panel: A[pend:, pend:oend] -= (Q S) Q^T
+                        // over its pend - p0 columns (rectangular; the upper part of the diagonal
+                        // block is not read)
+                        md.trans_b = true;
+                        for (int b = 0; b < nbatch; ++b) {
+                            const long long wq = pstride * b + (long long)pend * PO + (p0 - o0);
+                            md.add(make_prob(n - pend, oend - pend, pend - p0, wq + qs_off, PO, wq, PO,
+                                             strideA * b + (long long)pend * lda + pend, lda));
+                        }
+                        md.finalize(T128x128w8, s);
                     }
-                    tl.finalize(T128x128w8, s);
+                    const int mt = n - oend;
+                    if (j0 + nb == oend && mt > 0) {
+                        // trailing update of the outer panel: A22 -= (Q S) Q^T over its oend - o0 columns
+                        tl.trans_b = true;
+                        tl.lower_only = true;
+                        for (int b = 0; b < nbatch; ++b) {
+                            const long long wq = pstride * b + (long long)oend * PO;
+                            tl.add(make_prob(mt, mt, oend - o0, wq + qs_off, PO, wq, PO,
+                                             strideA * b + (long long)oend * lda + oend, lda));
+                        }
+                        tl.finalize(T128x128w8, s);
+                    }
+                    cp.trsm.push_back(std::move(tr));
+                    cp.inpanel.push_back(std::move(ip));
+                    cp.mid.push_back(std::move(md));
+                    cp.trail.push_back(std::move(tl));
                 }
-                cp.trsm.push_back(std::move(tr));
-                cp.inpanel.push_back(std::move(ip));
-                cp.trail.push_back(std::move(tl));
             }
         }
         it = plans.emplace(key, std::move(cp)).first;
@@ -638,23 +660,24 @@ void cholesky_logdet_batched(double *A, long long strideA, int nbatch, int n, in
     double *panel = w.panel.as<double>();
     for (size_t b = 0; b < cp.j0s.size(); ++b) {
         const int j0 = cp.j0s[b], nb = cp.nbs[b];
-        const int p0 = (j0 / CHOL_PW) * CHOL_PW;
+        const int o0 = (j0 / PO) * PO;
         hipLaunchKernelGGL(potrf_diag_kernel, dim3(nbatch), dim3(CHOL_NB), 0, s, A, strideA, lda, j0, nb, work,
                            logdet, ld_stride, flag, signs, sstride);
         HIP_CHECK(hipGetLastError());
         const int m = n - j0 - nb;
         if (m > 0) {
-            double *wq = panel + (long long)(j0 + nb) * CHOL_PW + (j0 - p0);
+            double *wq = panel + (long long)(j0 + nb) * PO + (j0 - o0);
             gemm_run(Prec::F64, cp.trsm[b], A, work, panel, 1.0, 0.0, s);
             hipLaunchKernelGGL(scale_cols_kernel, dim3(ceil_div((long long)m * nb, 256), nbatch), dim3(256), 0, s, wq,
-                               wq + qs_off, pstride, work, m, nb);
+                               wq + qs_off, pstride, PO, work, m, nb);
             HIP_CHECK(hipGetLastError());
             if (keep_L)
                 for (int bb = 0; bb < nbatch; ++bb)
-                    launch_copy_block(wq + pstride * bb + qs_off, CHOL_PW, A + strideA * bb + (long long)(j0 + nb) * lda + j0,
+                    launch_copy_block(wq + pstride * bb + qs_off, PO, A + strideA * bb + (long long)(j0 + nb) * lda + j0,
                                       lda, m, nb, s);
             gemm_run(Prec::F64, cp.inpanel[b], panel, panel, A, -1.0, 1.0, s);
         }
+        gemm_run(Prec::F64, cp.mid[b], panel, panel, A, -1.0, 1.0, s);
         gemm_run(Prec::F64, cp.trail[b], panel, panel, A, -1.0, 1.0, s);
     }
     if (keep_L)
