@@ -17,8 +17,12 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librpa_mi355x.so")
 ARCH = os.environ.get("LIBRPA_AMD_ARCH", "gfx950")
+# -amdgpu-mfma-vgpr-form: MFMA results in ordinary VGPRs.  Without it the 4-wave GEMM tiles keep
+# their accumulators in VGPRs across the K loop and copy all of them to AGPRs and back around the
+# MFMAs of every K slice (64 + 64 v_accvgpr moves per 16 MFMAs in the 64x128 T_A tile).
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wno-unused-command-line-argument"]
 
 
