@@ -765,17 +765,28 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         }
         prof.end("chi0_tau_to_omega", stream, 0, (double)E_tot * g.n * (ts * ceil_div(g.n, 8) + 8.0));
     }
-    // all omega of a q are factorised together, as far as memory allows
+    // all omega of a q are factorised together, as far as memory allows; small matrices (k-point
+    // workloads) also take several q per pass, so that the GEMM, Cholesky and log-det launches hold
+    // enough matrices to fill the device
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const double per_w = (double)nn * (3 * es + (f64 ? 0 : 8));
+    const double panel_w = 16.0 * n2 * (n2 >= 4096 ? 2048 : 512);  // Cholesky panels per matrix
+    const double per_w = (double)nn * (3 * es + (f64 ? 0 : 8)) + panel_w;
+    const double per_q = (double)nn * 12 + panel_w;
     const double budget = std::min(96e9, 0.6 * (double)free_b) - (double)nn * 12;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
-    DevBuf dV(nn * 8), dL32, dD(nn * es * WCH), dX(nn * es * WCH), dM(nn * es * WCH), dM64, dacc;
+    int QG = 1;
+    if (WCH == nw && nq > 1) {
+        QG = std::max(1, std::min(nq, (int)(budget / (nw * per_w + per_q))));
+        if (const char *qg = std::getenv("LIBRPA_AMD_EC_QGROUP")) QG = std::max(1, std::min(QG, std::atoi(qg)));
+    }
+    const int UCH = QG > 1 ? QG * nw : WCH;  // (q, omega) matrices per pass
+    DevBuf dV(nn * 8 * QG), dL32, dD(nn * es * UCH), dX(nn * es * UCH), dM(nn * es * UCH), dM64, dacc,
+        dVld(sizeof(double) * QG);
     // pivot signs S of V = L S L^T: a Coulomb matrix that is not positive definite keeps
     // det(1 - V chi0) = det(S) det(S - L^T chi0 L) and Tr(V chi0) = n - sum_i S_i M_ii with
     // M = S - L^T chi0 L, so Ec is exact for any V (as the LU of epsilon.cpp:600-693 is)
-    DevBuf dS((size_t)n2 * 8);
-    if (!f64) { dL32.alloc(nn * 4); dM64.alloc(nn * 8 * WCH); }
+    DevBuf dS((size_t)n2 * 8 * QG);
+    if (!f64) { dL32.alloc(nn * 4 * QG); dM64.alloc(nn * 8 * UCH); }
     dacc.alloc(sizeof(double) * 2 * nq * nw + 64);
     HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
     int *dflag = reinterpret_cast<int *>(dacc.as<char>() + sizeof(double) * 2 * nq * nw);
@@ -800,97 +811,112 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         HIP_CHECK(hipStreamSynchronize(stream));
         v_ibz = ibz;
     }
-    // (chunk size, order, precision) -> (X = D L, M = S - L^T X), kept with their tuned tiles
-    auto get_batches = [&](int cnt) -> std::pair<GemmBatch, GemmBatch> & {
-        const auto key = std::make_tuple(cnt, n2, (int)ec_prec);
+    // (q per pass, omega per q, order, precision) -> (X = D L, M = S - L^T X) over the pass's
+    // matrices u = qi * cnt + w, whose L is that of q qi; kept with their tuned tiles
+    auto get_batches = [&](int qg, int cnt) -> std::pair<GemmBatch, GemmBatch> & {
+        const auto key = std::make_tuple(qg, cnt, n2, (int)ec_prec);
         auto it = ec_batches.find(key);
         if (it != ec_batches.end()) return it->second;
         auto &pr = ec_batches[key];
         GemmBatch &bx = pr.first, &bm = pr.second;
         bm.trans_a = true;
         bm.lower_only = true;
-        for (int w = 0; w < cnt; ++w) {
-            GemmProb px = make_prob(n2, n2, n2, (long long)w * nn, n2, 0, n2, (long long)w * nn, n2);
-            px.kmode = 1;  // L is lower triangular: column tile n0 needs k >= n0 only
-            bx.add(px);
-            GemmProb pm = make_prob(n2, n2, n2, 0, n2, (long long)w * nn, n2, (long long)w * nn, n2);
-            pm.kmode = 2;  // A = L^T: row tile m0 needs k >= m0 only
-            bm.add(pm);
-        }
+        for (int qi = 0; qi < qg; ++qi)
+            for (int w = 0; w < cnt; ++w) {
+                const long long u = (long long)qi * cnt + w;
+                GemmProb px = make_prob(n2, n2, n2, u * nn, n2, (long long)qi * nn, n2, u * nn, n2);
+                px.kmode = 1;  // L is lower triangular: column tile n0 needs k >= n0 only
+                bx.add(px);
+                GemmProb pm = make_prob(n2, n2, n2, (long long)qi * nn, n2, u * nn, n2, u * nn, n2);
+                pm.kmode = 2;  // A = L^T: row tile m0 needs k >= m0 only
+                bm.add(pm);
+            }
         const int cfg = n2 >= 1024 ? big_tile(ec_prec) : T64x64;
         bx.finalize(cfg, stream);
         bm.finalize(cfg, stream);
         // executed flops of the triangular products
-        bx.flops = 2.0 * cnt * (double)n2 * n2 * n2 / 2.0;
-        bm.flops = 2.0 * cnt * (double)n2 * n2 * n2 / 6.0;
+        bx.flops = 2.0 * qg * cnt * (double)n2 * n2 * n2 / 2.0;
+        bm.flops = 2.0 * qg * cnt * (double)n2 * n2 * n2 / 6.0;
         return pr;
     };
-    for (int iq = 0; iq < nq; ++iq) {
-        const int kq = ibz[iq];
+    auto set_m_identity = [&](int qg, int cnt) {
+        for (int qi = 0; qi < qg; ++qi)
+            for (int w = 0; w < cnt; ++w) {
+                const size_t u = (size_t)qi * cnt + w;
+                const double *sg = dS.as<double>() + (size_t)qi * n2;
+                if (f64) launch_set_identity<double>(dM.as<double>() + u * nn, n2, n2, stream, sg);
+                else launch_set_identity<float>(dM.as<float>() + u * nn, n2, n2, stream, sg);
+            }
+    };
+    for (int iq0 = 0; iq0 < nq; iq0 += QG) {
+        const int qg = std::min(QG, nq - iq0);
         if (verbose) {
             HIP_CHECK(hipStreamSynchronize(stream));
-            fprintf(stderr, "[librpa-mi355x] Ec: q %d / %d (%.1f s)\n", iq + 1, nq, now_s() - t0);
+            fprintf(stderr, "[librpa-mi355x] Ec: q %d-%d / %d (%.1f s)\n", iq0 + 1, iq0 + qg, nq, now_s() - t0);
             fflush(stderr);
         }
         // dense V(q) from the staged upper atom-pair blocks (Hermitian fill, utils_matrix_m_mpi.h:430-441)
         prof.begin("ec_v_fill", stream);
-        HIP_CHECK(hipMemsetAsync(dV.get(), 0, nn * 8, stream));
-        launch_v_scatter(d_vblk.as<double>(), d_vdesc.as<VBlock>() + vdesc_q[iq], vdesc_q[iq + 1] - vdesc_q[iq],
-                         dV.as<double>(), n, n2, embed ? 1 : 0, 0, stream);
-        prof.end("ec_v_fill", stream, 0, (double)nn * 8);
+        HIP_CHECK(hipMemsetAsync(dV.get(), 0, (size_t)qg * nn * 8, stream));
+        for (int qi = 0; qi < qg; ++qi) {
+            const int iq = iq0 + qi;
+            launch_v_scatter(d_vblk.as<double>(), d_vdesc.as<VBlock>() + vdesc_q[iq], vdesc_q[iq + 1] - vdesc_q[iq],
+                             dV.as<double>() + (size_t)qi * nn, n, n2, embed ? 1 : 0, 0, stream);
+        }
+        prof.end("ec_v_fill", stream, 0, (double)qg * nn * 8);
         prof.begin("ec_chol_v", stream);
-        cholesky_logdet(dV.as<double>(), n2, n2, dacc.as<double>() + 2 * nq * nw + 4, dflag + 1, true, cw, stream,
-                        dS.as<double>());
-        prof.end("ec_chol_v", stream, (double)n2 * n2 * n2 / 3.0);
+        cholesky_logdet_batched(dV.as<double>(), (long long)nn, qg, n2, n2, dVld.as<double>(), 1, dflag + 1, true, cw,
+                                stream, -1, dS.as<double>(), n2);
+        prof.end("ec_chol_v", stream, (double)qg * n2 * n2 * n2 / 3.0);
         const void *Lp = dV.get();
         if (!f64) {
-            launch_convert<double, float>(dV.as<double>(), dL32.as<float>(), (long long)nn, stream);
+            launch_convert<double, float>(dV.as<double>(), dL32.as<float>(), (long long)qg * nn, stream);
             Lp = dL32.get();
         }
-        const double *kf = &ds.kfrac[(size_t)kq * 3];
-        for (int w0 = 0; w0 < nw; w0 += WCH) {
-            const int cnt = std::min(WCH, nw - w0);
+        const int cstep = qg > 1 ? nw : WCH;
+        for (int w0 = 0; w0 < nw; w0 += cstep) {
+            const int cnt = std::min(cstep, nw - w0);
+            const int nu = qg * cnt;
             // tau -> omega (formed once above, or per chunk) and R -> q of chi0s into dense matrices
             prof.begin("chi0_transform", stream);
-            assemble_dense(g, kf, w0, cnt, dD.get(), f64, (long long)nn, embed, stream);
-            const double tbytes = chiw_ok ? 8.0 * E_tot * cnt + (double)es * cnt * nn
-                                          : (double)ts * E_tot * ntau_done * ceil_div(cnt, kChi0AssembleMaxW) +
-                                                (double)es * cnt * nn;
+            for (int qi = 0; qi < qg; ++qi) {
+                const double *kf = &ds.kfrac[(size_t)ibz[iq0 + qi] * 3];
+                assemble_dense(g, kf, w0, cnt, static_cast<char *>(dD.get()) + (size_t)qi * cnt * nn * es, f64,
+                               (long long)nn, embed, stream);
+            }
+            const double tbytes = chiw_ok ? (8.0 * E_tot * cnt + (double)es * cnt * nn) * qg
+                                          : ((double)ts * E_tot * ntau_done * ceil_div(cnt, kChi0AssembleMaxW) +
+                                             (double)es * cnt * nn) * qg;
             prof.end("chi0_transform", stream, 0, tbytes);
-            auto &bb = get_batches(cnt);
+            auto &bb = get_batches(qg, cnt);
             if (autotune && !bb.first.tuned) {
                 bb.first.tune(ec_prec, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
                 gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
-                bb.second.tune(ec_prec, Lp, dX.get(), dM.get(), -1.0, 1.0, stream, [&]() {
-                    for (int w = 0; w < cnt; ++w) {
-                        if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
-                        else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
-                    }
-                });
+                bb.second.tune(ec_prec, Lp, dX.get(), dM.get(), -1.0, 1.0, stream, [&]() { set_m_identity(qg, cnt); });
             }
             prof.begin("ec_gemm_x", stream);
             gemm_run(ec_prec, bb.first, dD.get(), Lp, dX.get(), 1.0, 0.0, stream);
             prof.end("ec_gemm_x", stream, bb.first.flops);
-            for (int w = 0; w < cnt; ++w) {
-                if (f64) launch_set_identity<double>(dM.as<double>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
-                else launch_set_identity<float>(dM.as<float>() + (size_t)w * nn, n2, n2, stream, dS.as<double>());
-            }
+            set_m_identity(qg, cnt);
             prof.begin("ec_gemm_m", stream);
             gemm_run(ec_prec, bb.second, Lp, dX.get(), dM.get(), -1.0, 1.0, stream);
             prof.end("ec_gemm_m", stream, bb.second.flops);
             double *M64 = dM.as<double>();
             if (!f64) {
-                for (int w = 0; w < cnt; ++w)
-                    launch_copy_lower_f2d(dM.as<float>() + (size_t)w * nn, dM64.as<double>() + (size_t)w * nn, n2, n2,
+                for (int u = 0; u < nu; ++u)
+                    launch_copy_lower_f2d(dM.as<float>() + (size_t)u * nn, dM64.as<double>() + (size_t)u * nn, n2, n2,
                                           stream);
                 M64 = dM64.as<double>();
             }
-            double *acc0 = dacc.as<double>() + 2 * ((size_t)iq * nw + w0);
+            // accumulators of (q, omega) at 2 (q nw + omega): the pass's matrices are consecutive
+            // there (one q per pass, or all omega of each q)
+            double *acc0 = dacc.as<double>() + 2 * ((size_t)iq0 * nw + w0);
             prof.begin("ec_logdet", stream);
-            for (int w = 0; w < cnt; ++w)
-                launch_trace_acc(M64 + (size_t)w * nn, n2, n2, acc0 + 2 * w, stream, dS.as<double>());
-            cholesky_logdet_batched(M64, (long long)nn, cnt, n2, n2, acc0 + 1, 2, dflag, false, cw, stream);
-            prof.end("ec_logdet", stream, (double)cnt * n2 * n2 * n2 / 3.0);
+            for (int u = 0; u < nu; ++u)
+                launch_trace_acc(M64 + (size_t)u * nn, n2, n2, acc0 + 2 * u, stream,
+                                 dS.as<double>() + (size_t)(u / cnt) * n2);
+            cholesky_logdet_batched(M64, (long long)nn, nu, n2, n2, acc0 + 1, 2, dflag, false, cw, stream);
+            prof.end("ec_logdet", stream, (double)nu * n2 * n2 * n2 / 3.0);
         }
     }
     std::vector<double> hacc(2 * (size_t)nq * nw);

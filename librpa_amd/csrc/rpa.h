@@ -105,7 +105,7 @@ struct RpaEngine {
     int ntau_done = 0;
     int planned_ntau = 0;
     bool autotune = true;  // pick each batch's tile shape by timing on first use
-    std::map<std::tuple<int, int, int>, std::pair<GemmBatch, GemmBatch>> ec_batches;
+    std::map<std::tuple<int, int, int, int>, std::pair<GemmBatch, GemmBatch>> ec_batches;
 
     // ---- block-norm screening of chi0 outputs (nspin = 1) ----
     int screen_mode = 0;          // 0 off, 1 norm-product bound, 2 learned MLP and the bound
