@@ -114,8 +114,13 @@ void prepare_grid(Dataset &d, const LibrpaOptions &o) {
         emax = o.minimax_emax > 0 ? o.minimax_emax : emax_mf;
         emin = o.minimax_emin > 0 ? o.minimax_emin : emin_mf;
     }
+    const std::vector<double> key = {(double)o.tfgrids_type, (double)o.nfreq, emin, emax, o.minimax_regulation,
+                                     o.tfgrids_freq_min, o.tfgrids_freq_interval, o.tfgrids_time_min,
+                                     o.tfgrids_time_interval};
+    if (key == d.grid_key && d.grid.n == o.nfreq) return;
     d.grid = make_grid((int)o.tfgrids_type, o.nfreq, emin, emax, o.minimax_regulation, o.tfgrids_freq_min,
                        o.tfgrids_freq_interval, o.tfgrids_time_min, o.tfgrids_time_interval);
+    d.grid_key = key;
 }
 
 std::string json_escape(const std::string &s) {

@@ -75,6 +75,10 @@ struct Dataset {
 
     // results of the last compute calls
     TFGrid grid;
+    // the parameters grid was made from (type, n, e_min, e_max, regulation, frequency and time
+    // minima and intervals): the least-squares fits of the transform matrices take up to a second
+    // for 24 points, so an unchanged grid is not made again
+    std::vector<double> grid_key;
     std::shared_ptr<RpaEngine> rpa;
     std::shared_ptr<GwEngine> gw;
     double stats[10] = {0};

@@ -61,6 +61,13 @@ template <class TD>
 void launch_chi0_rq(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs, long long n_pair_elems,
                     const long long *eoff, int nR, const double *ph, int nw, TD *dense, long long dense_stride, int n,
                     bool embed, hipStream_t s);
+// chi0_rq for nq <= kChi0RqMaxQ q points at once (phases ph[q][r] as (cos, sin)); q point qi writes
+// its nw matrices at dense + qi * q_stride
+constexpr int kChi0RqMaxQ = 8;
+template <class TD>
+void launch_chi0_rq_multi(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs,
+                          long long n_pair_elems, const long long *eoff, int nR, const double *ph, int nq, int nw,
+                          TD *dense, long long dense_stride, long long q_stride, int n, bool embed, hipStream_t s);
 
 // Dense helpers --------------------------------------------------------------------
 // A = I, or diag(d) when d (n device doubles) is given

@@ -324,6 +324,84 @@ __global__ __launch_bounds__(256) void chi0_rq_kernel(const double *__restrict__
     }
 }
 
+// The same for up to kChi0RqMaxQ q points at once: each chiw value read feeds every q of the launch
+// (phases ph[q][r] as (cos, sin)); q point qi writes its nw matrices at dense + qi * qstride.
+template <class TD>
+__global__ __launch_bounds__(256) void chi0_rq_multi_kernel(const double *__restrict__ chiw, long long E_tot,
+                                                            const PairDesc *__restrict__ pairs, int npairs,
+                                                            long long nelem, const long long *__restrict__ eoff,
+                                                            int nR, const double *__restrict__ ph, int nq, int nw,
+                                                            TD *__restrict__ dense, long long dstride,
+                                                            long long qstride, int n, int embed) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nelem) return;
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].e0 <= idx) lo = mid; else hi = mid - 1;
+    }
+    const PairDesc P = pairs[lo];
+    const long long e = idx - P.e0;
+    const int mu = (int)(e / P.naJ), nu = (int)(e % P.naJ);
+    const long long row = P.rowI + mu, col = P.colJ + nu;
+    const bool mirror = P.I != P.J;
+    for (int w = 0; w < nw; ++w) {
+        double re[kChi0RqMaxQ], im[kChi0RqMaxQ];
+#pragma unroll
+        for (int q = 0; q < kChi0RqMaxQ; ++q) re[q] = im[q] = 0.0;
+        const double *cw = chiw + (long long)w * E_tot + e;
+        for (int r = 0; r < nR; ++r) {
+            const long long off = eoff[(long long)lo * nR + r];
+            if (off < 0) continue;
+            const double v = cw[off];
+#pragma unroll
+            for (int q = 0; q < kChi0RqMaxQ; ++q) {
+                if (q < nq) {
+                    re[q] += ph[2 * ((long long)q * nR + r)] * v;
+                    im[q] += ph[2 * ((long long)q * nR + r) + 1] * v;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kChi0RqMaxQ; ++q) {
+            if (q >= nq) break;
+            TD *D = dense + q * qstride + w * dstride;
+            if (!embed) {
+                D[row * n + col] = static_cast<TD>(re[q]);
+                if (mirror) D[col * n + row] = static_cast<TD>(re[q]);
+            } else {
+                const long long ld = 2ll * n;
+                D[row * ld + col] = static_cast<TD>(re[q]);
+                D[(n + row) * ld + n + col] = static_cast<TD>(re[q]);
+                D[(n + row) * ld + col] = static_cast<TD>(im[q]);
+                D[row * ld + n + col] = static_cast<TD>(-im[q]);
+                if (mirror) {
+                    D[col * ld + row] = static_cast<TD>(re[q]);
+                    D[(n + col) * ld + n + row] = static_cast<TD>(re[q]);
+                    D[(n + col) * ld + row] = static_cast<TD>(-im[q]);
+                    D[col * ld + n + row] = static_cast<TD>(im[q]);
+                }
+            }
+        }
+    }
+}
+
+template <class TD>
+void launch_chi0_rq_multi(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs,
+                          long long n_pair_elems, const long long *eoff, int nR, const double *ph, int nq, int nw,
+                          TD *dense, long long dense_stride, long long q_stride, int n, bool embed, hipStream_t s) {
+    if (nq < 1 || nq > kChi0RqMaxQ) LRA_THROW("chi0_rq_multi: 1 to kChi0RqMaxQ q points per launch");
+    hipLaunchKernelGGL((chi0_rq_multi_kernel<TD>), dim3(ceil_div(n_pair_elems, 256)), dim3(256), 0, s, chiw, E_tot,
+                       pairs, npairs, n_pair_elems, eoff, nR, ph, nq, nw, dense, dense_stride, q_stride, n, (int)embed);
+    HIP_CHECK(hipGetLastError());
+}
+template void launch_chi0_rq_multi<double>(const double *, long long, const PairDesc *, int, long long,
+                                           const long long *, int, const double *, int, int, double *, long long,
+                                           long long, int, bool, hipStream_t);
+template void launch_chi0_rq_multi<float>(const double *, long long, const PairDesc *, int, long long,
+                                          const long long *, int, const double *, int, int, float *, long long,
+                                          long long, int, bool, hipStream_t);
+
 template <class TD>
 void launch_chi0_rq(const double *chiw, long long E_tot, const PairDesc *pairs, int npairs, long long n_pair_elems,
                     const long long *eoff, int nR, const double *ph, int nw, TD *dense, long long dense_stride, int n,

@@ -879,12 +879,10 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             const int nu = qg * cnt;
             // tau -> omega (formed once above, or per chunk) and R -> q of chi0s into dense matrices
             prof.begin("chi0_transform", stream);
-            for (int qi = 0; qi < qg; ++qi) {
-                const double *kf = &ds.kfrac[(size_t)ibz[iq0 + qi] * 3];
-                assemble_dense(g, kf, w0, cnt, static_cast<char *>(dD.get()) + (size_t)qi * cnt * nn * es, f64,
-                               (long long)nn, embed, stream);
-            }
-            const double tbytes = chiw_ok ? (8.0 * E_tot * cnt + (double)es * cnt * nn) * qg
+            std::vector<const double *> kfs(qg);
+            for (int qi = 0; qi < qg; ++qi) kfs[qi] = &ds.kfrac[(size_t)ibz[iq0 + qi] * 3];
+            assemble_dense_q(g, kfs, w0, cnt, dD.get(), f64, (long long)nn, embed, stream);
+            const double tbytes = chiw_ok ? 8.0 * E_tot * cnt * ceil_div(qg, kChi0RqMaxQ) + (double)es * cnt * nn * qg
                                           : ((double)ts * E_tot * ntau_done * ceil_div(cnt, kChi0AssembleMaxW) +
                                              (double)es * cnt * nn) * qg;
             prof.end("chi0_transform", stream, 0, tbytes);
@@ -971,6 +969,41 @@ bool RpaEngine::form_chiw(const TFGrid &g, double max_bytes, hipStream_t s) {
     HIP_CHECK(hipStreamSynchronize(s));  // dc is freed on return
     chiw_ok = true;
     return true;
+}
+
+void RpaEngine::assemble_dense_q(const TFGrid &g, const std::vector<const double *> &kfs, int w0, int cnt,
+                                 void *dense, bool f64, long long nn, bool embed, hipStream_t s) {
+    const size_t es = f64 ? 8 : 4;
+    if (!chiw_ok) {
+        for (size_t qi = 0; qi < kfs.size(); ++qi)
+            assemble_dense(g, kfs[qi], w0, cnt, static_cast<char *>(dense) + qi * cnt * nn * es, f64, nn, embed, s);
+        return;
+    }
+    const int n = NABF;
+    for (size_t q0 = 0; q0 < kfs.size(); q0 += kChi0RqMaxQ) {
+        const int nq = (int)std::min<size_t>(kChi0RqMaxQ, kfs.size() - q0);
+        std::vector<double> ph((size_t)nq * nR * 2);
+        for (int qi = 0; qi < nq; ++qi) {
+            const double *kf = kfs[q0 + qi];
+            for (int r = 0; r < nR; ++r) {
+                const double ang = 2 * M_PI * (kf[0] * cells[r][0] + kf[1] * cells[r][1] + kf[2] * cells[r][2]);
+                ph[((size_t)qi * nR + r) * 2] = std::cos(ang);
+                ph[((size_t)qi * nR + r) * 2 + 1] = std::sin(ang);
+            }
+        }
+        DevBuf dph = upload(ph, s);
+        const double *cw = d_chiw.as<double>() + (size_t)w0 * E_tot;
+        char *Dd = static_cast<char *>(dense) + q0 * cnt * nn * es;
+        if (f64)
+            launch_chi0_rq_multi<double>(cw, E_tot, d_pairs.as<PairDesc>(), (int)pairs.size(), n_pair_elems,
+                                         d_pair_eoff.as<long long>(), nR, dph.as<double>(), nq, cnt, (double *)Dd, nn,
+                                         (long long)cnt * nn, n, embed, s);
+        else
+            launch_chi0_rq_multi<float>(cw, E_tot, d_pairs.as<PairDesc>(), (int)pairs.size(), n_pair_elems,
+                                        d_pair_eoff.as<long long>(), nR, dph.as<double>(), nq, cnt, (float *)Dd, nn,
+                                        (long long)cnt * nn, n, embed, s);
+        HIP_CHECK(hipStreamSynchronize(s));  // dph is freed on return
+    }
 }
 
 void RpaEngine::assemble_dense(const TFGrid &g, const double *kf, int w0, int cnt, void *dense, bool f64, long long nn,

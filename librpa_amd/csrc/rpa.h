@@ -149,6 +149,11 @@ struct RpaEngine {
     // elements (FP64 or FP32), real at Gamma or the real embedding
     void assemble_dense(const TFGrid &g, const double *kf, int w0, int cnt, void *dense, bool f64, long long nn,
                         bool embed, hipStream_t s);
+    // the same for several q points at once (chi0(R, omega) formed): q point qi of kfs writes its
+    // cnt matrices at dense + qi * cnt * nn elements; chi0(R, omega) is read once per
+    // kChi0RqMaxQ q points
+    void assemble_dense_q(const TFGrid &g, const std::vector<const double *> &kfs, int w0, int cnt, void *dense,
+                          bool f64, long long nn, bool embed, hipStream_t s);
     long long ghat_max = 0, t_max = 0;
     RpaResult ec(const Dataset &ds, const TFGrid &g);
     // dense chi0(q, omega) block for verification
