@@ -443,8 +443,8 @@ def main():
     n64 = max(3, args.steps)
     ec64, t64, prof64, st64 = run_steps(L, h, o, s.nk, L.FP64, n64, max(1, args.warmup))
     s64 = float(np.median(t64))
-    _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step (median of {n64})  chi0 {st64['t_chi0']:.3f}s  "
-         f"Ec {st64['t_ec']:.3f}s")
+    _log(f"fp64: Ec {ec64:.10f}  {s64:.3f} s/step (median of {n64}: {' '.join('%.3f' % t for t in t64)})  "
+         f"chi0 {st64['t_chi0']:.3f}s  Ec {st64['t_ec']:.3f}s (last step)")
 
     # (chi0 mode, Ec transform precision) in the order of their speed; the first that meets the Ec bar
     # is timed
@@ -469,7 +469,8 @@ def main():
         if dec <= EC_TOL_PER_ATOM or args.prec != "auto":
             ec, times, prof, st = run_steps(L, h, o, s.nk, L.PREC[pn], args.steps, args.warmup, L.PREC[ecn])
             dec = abs(ec - ec64) / s.natom
-            _log(f"{pn} (Ec transforms {ecn}): Ec {ec:.10f}  dEc/atom {dec:.3e}  {np.median(times):.3f} s/step")
+            _log(f"{pn} (Ec transforms {ecn}): Ec {ec:.10f}  dEc/atom {dec:.3e}  {np.median(times):.3f} s/step "
+                 f"({' '.join('%.3f' % t for t in times)})")
             chosen = (pn, ecn, ec, times, prof, st, dec)
             break
     if chosen is None:

@@ -262,9 +262,39 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
         }
     }
 
-    // epilogue: acc[a][b][r] holds C[row][n0 + wn*WTN + b*16 + lane%16]
+    // epilogue: acc[a][b][r] holds C[row][n0 + wn*WTN + b*16 + lane%16].  The output column of
+    // each b (column split and table included) depends on the lane only: worked out once here,
+    // not for every row (a split's division by ncb is some thirty VALU instructions)
     const bool split = P.ncb < P.N;
     const bool ct = VAR == 2 && P.ct_off >= 0;
+    long long ccol[FN];
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+        const int gn = n0 + wn * WTN + b * 16 + (lane & 15);
+        ccol[b] = -1;
+        if (gn < P.N) {
+            const int inner = split ? gn % P.ncb : gn;
+            long long col = ct ? coltab[P.ct_off + inner] : inner;
+            if (split) col += (long long)(gn / P.ncb) * P.cbs;
+            ccol[b] = col;
+        }
+    }
+    // a full tile with alpha = 1, beta = 0 and plain stores (the chi0 build's W, X and T_A GEMMs):
+    // no per-element checks, scaling or reads of C
+    const bool plain = !((VAR == 2 || VAR == 3) && atomic) && alpha == T(1) && beta == T(0);
+    if (plain && m0 + BM <= P.M && n0 + BN <= P.N) {
+#pragma unroll
+        for (int a = 0; a < FM; ++a) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * WTM + a * 16 + (MODE == 0 ? 4 * r + kq : 4 * kq + r);
+                const long long crow = rt ? rowtab[2ll * (P.rt_off + gm) + 1] : P.c_off + (long long)gm * P.ldc;
+#pragma unroll
+                for (int b = 0; b < FN; ++b) C[crow + ccol[b]] = acc[a][b][r];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < FM; ++a) {
 #pragma unroll
@@ -276,12 +306,8 @@ __global__ __launch_bounds__(WM * WN * 64) void ggemm_kernel(const GemmProb *__r
             const long long crow = rt ? rowtab[2ll * (P.rt_off + gm) + 1] : P.c_off + (long long)gm * P.ldc;
 #pragma unroll
             for (int b = 0; b < FN; ++b) {
-                const int gn = n0 + wn * WTN + b * 16 + (lane & 15);
-                if (gn >= P.N) continue;
-                const int inner = split ? gn % P.ncb : gn;
-                long long col = ct ? coltab[P.ct_off + inner] : inner;
-                if (split) col += (long long)(gn / P.ncb) * P.cbs;
-                T *dst = C + crow + col;
+                if (ccol[b] < 0) continue;
+                T *dst = C + crow + ccol[b];
                 const T v0 = alpha * acc[a][b][r];
                 if ((VAR == 2 || VAR == 3) && atomic) {
                     atomicAdd(dst, v0);

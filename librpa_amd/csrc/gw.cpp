@@ -1142,6 +1142,7 @@ void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
     }
     eng->build_chi0(d, d.grid, o.n_bands_chi0);
     eng->release_work();  // chi0(R, tau) stays; the chi0 work buffers are rebuilt on demand
+    eng->release_ec();    // an earlier Ec evaluation's buffers
     auto gw = std::make_shared<GwEngine>();
     gw->verbose = o.output_level >= LIBRPA_VERBOSE_DEBUG;
     gw->head = head;

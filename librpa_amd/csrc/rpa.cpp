@@ -769,10 +769,15 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
     // workloads) also take several q per pass, so that the GEMM, Cholesky and log-det launches hold
     // enough matrices to fill the device
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    // the Ec buffers kept from an earlier call count as free here (they are reused or regrown)
+    double held = 0;
+    for (const DevBuf *b : {&ec_V, &ec_L32, &ec_D, &ec_X, &ec_M, &ec_M64, &ec_S, &ec_Vld, &ec_acc, &ec_cw.linv,
+                            &ec_cw.panel})
+        held += (double)b->bytes();
     const double panel_w = 16.0 * n2 * (n2 >= 4096 ? 2048 : 512);  // Cholesky panels per matrix
     const double per_w = (double)nn * (3 * es + (f64 ? 0 : 8)) + panel_w;
     const double per_q = (double)nn * 12 + panel_w;
-    const double budget = std::min(96e9, 0.6 * (double)free_b) - (double)nn * 12;
+    const double budget = std::min(96e9, 0.6 * ((double)free_b + held)) - (double)nn * 12;
     const int WCH = std::max(1, std::min(nw, (int)(budget / per_w)));
     int QG = 1;
     if (WCH == nw && nq > 1) {
@@ -780,17 +785,23 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
         if (const char *qg = std::getenv("LIBRPA_AMD_EC_QGROUP")) QG = std::max(1, std::min(QG, std::atoi(qg)));
     }
     const int UCH = QG > 1 ? QG * nw : WCH;  // (q, omega) matrices per pass
-    DevBuf dV(nn * 8 * QG), dL32, dD(nn * es * UCH), dX(nn * es * UCH), dM(nn * es * UCH), dM64, dacc,
-        dVld(sizeof(double) * QG);
+    DevBuf &dV = ec_V, &dL32 = ec_L32, &dD = ec_D, &dX = ec_X, &dM = ec_M, &dM64 = ec_M64, &dacc = ec_acc,
+           &dVld = ec_Vld, &dS = ec_S;
+    dV.reserve(nn * 8 * QG);
+    dD.reserve(nn * es * UCH);
+    dX.reserve(nn * es * UCH);
+    dM.reserve(nn * es * UCH);
+    dVld.reserve(sizeof(double) * QG);
     // pivot signs S of V = L S L^T: a Coulomb matrix that is not positive definite keeps
     // det(1 - V chi0) = det(S) det(S - L^T chi0 L) and Tr(V chi0) = n - sum_i S_i M_ii with
     // M = S - L^T chi0 L, so Ec is exact for any V (as the LU of epsilon.cpp:600-693 is)
-    DevBuf dS((size_t)n2 * 8 * QG);
-    if (!f64) { dL32.alloc(nn * 4 * QG); dM64.alloc(nn * 8 * UCH); }
-    dacc.alloc(sizeof(double) * 2 * nq * nw + 64);
-    HIP_CHECK(hipMemsetAsync(dacc.get(), 0, dacc.bytes(), stream));
+    dS.reserve((size_t)n2 * 8 * QG);
+    if (!f64) { dL32.reserve(nn * 4 * QG); dM64.reserve(nn * 8 * UCH); }
+    const size_t acc_bytes = sizeof(double) * 2 * nq * nw + 64;
+    dacc.reserve(acc_bytes);
+    HIP_CHECK(hipMemsetAsync(dacc.get(), 0, acc_bytes, stream));
     int *dflag = reinterpret_cast<int *>(dacc.as<char>() + sizeof(double) * 2 * nq * nw);
-    CholWork cw;
+    CholWork &cw = ec_cw;
     if (v_ibz != ibz) {
         // stage the input's V(q) blocks once per plan (the engine is rebuilt when inputs change)
         std::vector<VBlock> desc;
@@ -934,6 +945,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
             res.qcontrib[iq] += cplx(val * g.omega_w[iw] * wq[iq] / (2 * M_PI), 0.0);
         }
     for (auto &c : res.qcontrib) res.ec_re += c.real();
+    // a host-staged system needs the device memory back for the next chi0 build
+    if (host_staged) release_ec();
     t_ec = now_s() - t0;
     return res;
 }
@@ -1058,6 +1071,11 @@ void RpaEngine::assemble_dense(const TFGrid &g, const double *kf, int w0, int cn
                                                dgp.as<double>(), cw_n, (float *)Dd, nn, n, embed, s);
         HIP_CHECK(hipStreamSynchronize(s));  // dgp is freed on return
     }
+}
+
+void RpaEngine::release_ec() {
+    for (DevBuf *b : {&ec_V, &ec_L32, &ec_D, &ec_X, &ec_M, &ec_M64, &ec_S, &ec_Vld, &ec_acc, &ec_cw.linv, &ec_cw.panel})
+        b->release();
 }
 
 void RpaEngine::release_work() {

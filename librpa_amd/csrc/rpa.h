@@ -97,6 +97,11 @@ struct RpaEngine {
     // the first Ec call (block store, descriptors, first descriptor of each q)
     DevBuf d_vblk, d_vdesc;
     std::vector<int> vdesc_q, v_ibz;
+    // Ec work buffers, kept between evaluations (allocating and freeing ~100 GB per call made
+    // some calls 50% slower): V / L and pivot signs per q, chi0(q, omega), X, M per matrix, the
+    // Cholesky workspace
+    DevBuf ec_V, ec_L32, ec_D, ec_X, ec_M, ec_M64, ec_S, ec_Vld, ec_acc;
+    CholWork ec_cw;
 
     // counters
     double flops_chi0_alg = 0;   // canonical count per spin, all tau
@@ -138,6 +143,7 @@ struct RpaEngine {
     void build_gf(const MeanField &mf, double tau, int s);
     void build_chi0(const Dataset &ds, const TFGrid &g, int nbands_g);
     void release_work();  // free the gathered-G, W and T work buffers (rebuilt by build_chi0)
+    void release_ec();    // free the Ec work buffers (rebuilt by ec)
     // chi0(R, omega) = sum_tau (2 / nspin) gamma_c(omega, tau) chi0(R, tau) of the last build, FP64
     // [omega][E], formed once per Ec / W evaluation when it fits in max_bytes (returns whether it
     // was formed); the R -> q sums then read it once per q instead of chi0(R, tau) per chunk of 8
