@@ -10,6 +10,8 @@
   eigenvalues in v^{1/2}; test_gpu_gw_special.py covers it.)
 * split-K: the contraction's atomically accumulated K parts agree with one plain write to
   1e-13 in FP64, and one plain write is bitwise repeatable.
+* the host-staged chi0 store gives the device store's Ec and blocks; Ec with several q per pass
+  gives the per-q contributions of one q per pass.
 """
 import numpy as np
 import pytest
@@ -142,8 +144,8 @@ def test_split_k_parts_agree_and_single_part_is_bitwise_repeatable(monkeypatch):
 
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED])
 def test_host_staged_chi0_store_gives_the_same_result(monkeypatch, prec):
-    # chi0(R, tau) kept in pinned host memory slice by slice (what a store too large for the
-    # device does) and streamed back into chi0(R, omega) for Ec: the same Ec and blocks
+    # chi0(R, tau) kept in host memory slice by slice (what a store too large for the device
+    # does) and streamed back into chi0(R, omega) for Ec: the same Ec and blocks
     _say(f"host-staged chi0 store, prec {prec}")
     s = synth.cell2()
     taus = (0, 7, 15)
@@ -158,3 +160,21 @@ def test_host_staged_chi0_store_gives_the_same_result(monkeypatch, prec):
     assert worst == 0.0
     assert abs(ec1 - ec0) <= 1e-13 * abs(ec0)
     assert np.abs(c1 - c0).max() <= 1e-13 * abs(ec0)
+
+
+def test_ec_q_grouping_gives_the_same_result(monkeypatch):
+    # Ec factorises several q per pass on k-point workloads (batched V Cholesky, GEMMs and
+    # log-det; chi0(R, omega) read once per 8 q): one q per pass, and passes of 4 + 4 + 1 q, give
+    # the same per-q contributions as all 9 q in one pass
+    _say("Ec q grouping, 3x3 k")
+    s = synth.tiny(**KCASES["k3x3_larger"])
+    monkeypatch.setenv("LIBRPA_AMD_SPLITK", "1")
+    _, _, ec_all, c_all = _ec(s)
+    res = {}
+    for qg in ("1", "4"):
+        monkeypatch.setenv("LIBRPA_AMD_EC_QGROUP", qg)
+        _, _, ec, c = _ec(s)
+        res[qg] = (ec, c)
+        _say(f"  q per pass {qg}: Ec {ec!r} (all in one pass {ec_all!r})")
+        assert abs(ec - ec_all) <= 1e-12 * abs(ec_all)
+        assert np.abs(c - c_all).max() <= 1e-12 * abs(ec_all)
