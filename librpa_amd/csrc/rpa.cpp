@@ -954,7 +954,8 @@ RpaResult RpaEngine::ec(const Dataset &ds, const TFGrid &g) {
 bool RpaEngine::form_chiw(const TFGrid &g, double max_bytes, hipStream_t s) {
     chiw_ok = false;
     const double need = 8.0 * (double)E_tot * g.n;
-    if (ntau_done != g.n || g.n > kTauOmegaMax || need > max_bytes) return false;
+    // a store kept from an earlier evaluation is reused (its bytes are not in the free memory)
+    if (ntau_done != g.n || g.n > kTauOmegaMax || need > max_bytes + (double)d_chiw.bytes()) return false;
     d_chiw.reserve((size_t)need);
     std::vector<double> coef((size_t)g.n * g.n);
     for (int w = 0; w < g.n; ++w)
