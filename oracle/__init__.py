@@ -9,4 +9,9 @@ nothing in the shipped library calls into it.
 Every function cites the reference file:line it follows.  Pins (what fixes
 each function independently of this restatement) are listed in the module
 docstrings and exercised by ``tests/test_oracle_*.py``.
+
+``ref_libri.cpp`` and ``Makefile`` build ``_ref/ref_libri``, a checker that runs the
+reference's own header-only LibRI contractions (``RPA::cal_chi0s``, ``GW::cal_Sigmas``)
+from the reference source tree; ``tests/test_oracle_libri.py`` holds this package's label
+sums against it.
 """

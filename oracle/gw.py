@@ -46,6 +46,9 @@ Pins (tests/test_oracle_gw.py): W against the closed form v (1 - chi0 v)^{-1} - 
 fails); ``sigma_factorized`` against both; the transpose symmetry
 Sigma_XY(R) = Sigma_YX(-R)^T; Pade against -1/(x - x0) (src/test/test_analycont.cpp:
 15-46); the QP solver against the fixed point of a pole model solved in closed form.
+tests/test_oracle_libri.py: every block of ``sigma_labels`` equals the reference's own LibRI
+``GW::cal_Sigmas`` (oracle/ref_libri.cpp) for both signs of tau on k-point systems, to 1e-12
+relative, and the planted a1b1 cell mistake fails it.
 """
 from __future__ import annotations
 

@@ -22,6 +22,9 @@ Restates LibRPA's LibRI-routing path:
 Pins (tests/test_oracle_rpa.py): G(tau -> 0-) = -D (src/test/test_meanfield.cpp:43-56)
 and an explicit per-k double loop; brute force vs factorised chi0 on mixed-species
 k-point systems; transforms vs explicit loops; ln det vs eigenvalue sum; Ec <= 0.
+tests/test_oracle_libri.py: every block of ``chi0s_bruteforce`` equals the reference's own
+LibRI ``RPA::cal_chi0s`` (oracle/ref_libri.cpp, built from the reference tree by
+oracle/Makefile) on k-point systems with periods up to 3 x 2 x 1, to 1e-12 relative.
 """
 from __future__ import annotations
 
