@@ -1068,9 +1068,12 @@ PadeAC::PadeAC(int n_pars, const std::vector<cplx> &xs, const std::vector<cplx> 
         px = xs;
         pd = data;
     } else {
+        // n_pars - 1 evenly spaced nodes, then the last datum. LibRPA sizes its node list to
+        // n_pars and fills only the first n_pars - 1, so that datum sits at the node 0
+        // (src/core/analycont.cpp:35-45); the same node keeps the continuation identical.
         const int step = nd / (n_pars - 1);
         for (int i = 0; i < n_pars - 1; ++i) { px.push_back(xs[(size_t)i * step]); pd.push_back(data[(size_t)i * step]); }
-        px.push_back(xs[nd - 1]);
+        px.push_back(cplx(0.0, 0.0));
         pd.push_back(data[nd - 1]);
     }
     n = n_pars;

@@ -10,8 +10,9 @@ Every function cites the reference file:line it follows.  Pins (what fixes
 each function independently of this restatement) are listed in the module
 docstrings and exercised by ``tests/test_oracle_*.py``.
 
-``ref_libri.cpp`` and ``Makefile`` build ``_ref/ref_libri``, a checker that runs the
-reference's own header-only LibRI contractions (``RPA::cal_chi0s``, ``GW::cal_Sigmas``)
-from the reference source tree; ``tests/test_oracle_libri.py`` holds this package's label
-sums against it.
+``Makefile`` builds two checkers from the reference source tree into ``_ref/``:
+``ref_libri`` (``ref_libri.cpp``) runs the reference's own header-only LibRI contractions
+(``RPA::cal_chi0s``, ``GW::cal_Sigmas``), ``ref_qpe`` (``ref_qpe.cpp``) its Pade continuation
+and QP solver. ``tests/test_oracle_libri.py`` and ``tests/test_oracle_ref_qpe.py`` hold this
+package against them.
 """

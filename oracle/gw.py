@@ -49,6 +49,9 @@ Sigma_XY(R) = Sigma_YX(-R)^T; Pade against -1/(x - x0) (src/test/test_analycont.
 tests/test_oracle_libri.py: every block of ``sigma_labels`` equals the reference's own LibRI
 ``GW::cal_Sigmas`` (oracle/ref_libri.cpp) for both signs of tau on k-point systems, to 1e-12
 relative, and the planted a1b1 cell mistake fails it.
+tests/test_oracle_ref_qpe.py: ``Pade`` and ``qpe_solve`` equal the reference's own AnalyContPade and
+qpe_solver_pade_self_consistent (oracle/ref_qpe.cpp, compiled from the reference's sources) to
+1e-10 for every kind of parameter count, including the subsampled fit's last node 0.
 """
 from __future__ import annotations
 
@@ -407,8 +410,11 @@ class Pade:
             n_pars = nd
             px, pd = xs, data
         else:
+            # n_pars - 1 evenly spaced nodes, then the last datum; LibRPA sizes its node list to
+            # n_pars and fills only the first n_pars - 1, so that datum sits at the node 0
+            # (analycont.cpp:35-45)
             step = nd // (n_pars - 1)
-            px = [xs[i * step] for i in range(n_pars - 1)] + [xs[nd - 1]]
+            px = [xs[i * step] for i in range(n_pars - 1)] + [0j]
             pd = [data[i * step] for i in range(n_pars - 1)] + [data[nd - 1]]
         self.n = n_pars
         self.x = np.array(px, dtype=np.complex128)

@@ -142,17 +142,21 @@ def test_g0w0_parity_spin_polarised():
         assert num / den < 1e-10
 
 
+@pytest.mark.parametrize("n_params", [-1, 4])
 @pytest.mark.parametrize("case", ["tiny_k2_mixed", "cell2_k8"])
-def test_g0w0_quasiparticle(case):
+def test_g0w0_quasiparticle(case, n_params):
     """Pade continuation and the QP fixed point on the library's Sigma_KS(i omega).
 
+    n_params = 4 (fewer parameters than frequencies) takes LibRPA's subsampled fit, whose last
+    datum sits at the node 0 (analycont.cpp:35-45).
     The Coulomb matrix is scaled by 1e-3 so that Sigma_c stays in a physical range (with the
     unscaled synthetic inputs Sigma_c reaches 1e8 Ha and the damped fixed point does not
     converge, for the reference as for this build)."""
-    _say(f"g0w0 quasi-particle case={case}")
+    _say(f"g0w0 quasi-particle case={case} n_params_anacon={n_params}")
     s = CASES[case]()
     s.vq = s.vq * 1e-3
     h, o, grid = _lib(s)
+    o.n_params_anacon = n_params
     nocc = int((s.occ[0, 0] > 0).sum())
     states = list(range(max(0, nocc - 4), min(s.nbands, nocc + 4)))
     iks = list(range(s.nk))
