@@ -10,9 +10,10 @@ Every function cites the reference file:line it follows.  Pins (what fixes
 each function independently of this restatement) are listed in the module
 docstrings and exercised by ``tests/test_oracle_*.py``.
 
-``Makefile`` builds two checkers from the reference source tree into ``_ref/``:
+``Makefile`` builds three checkers from the reference source tree into ``_ref/``:
 ``ref_libri`` (``ref_libri.cpp``) runs the reference's own header-only LibRI contractions
 (``RPA::cal_chi0s``, ``GW::cal_Sigmas``), ``ref_qpe`` (``ref_qpe.cpp``) its Pade continuation
-and QP solver. ``tests/test_oracle_libri.py`` and ``tests/test_oracle_ref_qpe.py`` hold this
-package against them.
+and QP solver, ``ref_minimax`` (``ref_minimax.f90``) its vendored greenX minimax generator.
+``tests/test_oracle_libri.py``, ``tests/test_oracle_ref_qpe.py`` and
+``tests/test_oracle_ref_minimax.py`` hold this package against them.
 """

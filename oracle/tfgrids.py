@@ -18,6 +18,11 @@ Pins: tests/golden/h2o_minimax16.json holds the 16-point grid and cosine /
 sine transform matrices LibRPA printed for the H2O regression case
 (regression_tests/refs/rpa_aims_mole_H2O_libri/librpa/librpa.out:129-224)
 together with that case's eigenvalues; test_timefreq.cpp:54-57 values (n=6).
+tests/test_oracle_ref_minimax.py: against the reference's own greenX gx_minimax_grid
+(oracle/ref_minimax.f90, compiled from the greenX sources) for all 15 sizes on four energy
+ranges and the regression cases' ranges: nodes and weights to 1e-14 (they come out identical),
+transform matrices to max(2e-13, eps * kappa) wherever the least-squares fit is well posed,
+and fit residuals as small as greenX's everywhere.
 """
 from __future__ import annotations
 
