@@ -11,8 +11,10 @@ The timed path is the fastest contraction precision whose Ec stays within
 1e-5 Ha/atom of the plain FP64 GPU path, which is timed beside it.
 
 Output: one JSON line (metric, value = s/freq, lower is better) with a
-`roofline` object for the dominant kernel and a `cpu_baseline` object (the FP64
-oracle timed on a bounded sample of the same workload, scaled).
+`roofline` object for the dominant kernel and a `cpu_baseline` object: the
+reference's own chi0 contraction (LibRI, from the checker built under
+oracle/_ref) timed on bounded atom samples of the same workload and scaled, with
+G and Ec from the FP64 oracle; without the checker, the FP64 oracle alone.
 """
 import argparse
 import json
@@ -162,7 +164,92 @@ def cpu_baseline_measured(L, reps=3):
             "ratio": round(t_cpu / t_gpu, 1), "ec_cpu_ha": ec_cpu, "ec_gpu_ha": ec_gpu}
 
 
+def _balanced_atoms(sys_, n):
+    """n atoms with the workload's mix of atom kinds, a kind being the (n_ao, n_abf) pair that sets
+    an atom's contraction cost: the first atoms of each kind, in numbers proportional to the
+    kind's share (largest remainders)."""
+    sig = [(int(a), int(b)) for a, b in zip(sys_.nao, sys_.nabf)]
+    kinds = sorted(set(sig))
+    members = {k: [i for i, g in enumerate(sig) if g == k] for k in kinds}
+    quota = {k: n * len(members[k]) / len(sig) for k in kinds}
+    take = {k: int(quota[k]) for k in kinds}
+    for k in sorted(kinds, key=lambda k: quota[k] - take[k], reverse=True)[:n - sum(take.values())]:
+        take[k] += 1
+    return sorted(i for k in kinds for i in members[k][:take[k]])
+
+
+def _ec_port_seconds(sys_, nfreq):
+    """numpy FP64 ln det(1 - V chi0) + GEMM at the full matrix size, times (nfreq x nk)."""
+    from oracle import rpa
+    m = int(sys_.nabf.sum())
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal((m, m))
+    V = b @ b.T / m + 1e-3 * np.eye(m)
+    x = rng.standard_normal((m, m))
+    chi = -(x @ x.T) / m
+    t0 = time.perf_counter()
+    rpa.ln_det_ref(np.eye(m) - V @ chi)
+    return (time.perf_counter() - t0) * nfreq * sys_.nk
+
+
+def cpu_baseline_reference(sys_, grid, nfreq):
+    """The reference's own chi0 contraction (LibRI RPA::cal_chi0s, compiled from the reference's
+    sources into oracle/_ref/ref_libri) timed at one tau on two species-balanced atom samples and
+    extrapolated to the whole cell with t(s) = a s + b s^2 (the output atom pairs grow as s^2);
+    G(+-tau) and Ec come from the numpy FP64 port, since LibRPA's G builder and ScaLAPACK Ec are
+    not compiled here."""
+    from oracle import ref_io, rpa
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    Rs = rpa.r_grid(sys_.period)
+    it = len(grid["tau"]) // 2
+    tau = grid["tau"][it]
+    t0 = time.perf_counter()
+    Gp = rpa.gf_R(sys_, tau, Rs)
+    Gn = rpa.gf_R(sys_, -tau, Rs)
+    t_g = time.perf_counter() - t0
+    n = sys_.natom
+    s1, s2 = max(2, n // 4), max(4, n // 2)
+    times = []
+    for s in (s1, s2):
+        atoms = _balanced_atoms(sys_, s)
+        _, dt = ref_io.chi0s(sys_, Gp, Gn, atoms=set(atoms), threads=threads, timeout=300, read=False)
+        times.append(dt)
+        _log(f"  reference chi0 (LibRI cal_chi0s) on {s} of {sys_.natom} atoms: {dt:.1f}s on {threads} threads")
+    (t1, t2) = times
+    # t(s) = a s + b s^2 through both samples; where start-up time dominates them (tiny systems)
+    # and the fit turns negative, the larger sample scaled by the pair count
+    b = (t2 / s2 - t1 / s1) / (s2 - s1)
+    a = t1 / s1 - b * s1
+    t_chi0 = a * n + b * n * n if a >= 0 and b >= 0 else t2 * (n / s2) ** 2
+    t_ec = _ec_port_seconds(sys_, nfreq)
+    total = (t_chi0 + t_g) * len(grid["tau"]) + t_ec
+    return dict(value=round(total / nfreq, 3), unit="s/freq", cores=threads, kind="reference",
+                extrapolated=True,
+                parts={"chi0_reference_s_per_tau": round(t_chi0, 2), "g_port_s_per_tau": round(t_g, 2),
+                       "ec_port_s": round(t_ec, 2),
+                       "libri_samples": {str(s1): round(t1, 2), str(s2): round(t2, 2)}},
+                sample=(f"LibRI cal_chi0s (the reference's chi0 contraction) at tau {it} on {s1} and {s2} "
+                        f"species-balanced atoms of {n}, fitted as a s + b s^2 and evaluated at {n} atoms, "
+                        f"x {len(grid['tau'])} tau; G(+-tau) and Ec (one ln det per omega and q at "
+                        f"n_abf) from the numpy FP64 port; total / nfreq"))
+
+
 def cpu_baseline(sys_, grid, nfreq):
+    """The reference's own chi0 contraction where its checker is built (cpu_baseline_reference),
+    else the FP64 port (cpu_baseline_port)."""
+    from oracle import ref_io
+    if ref_io.available():
+        try:
+            return cpu_baseline_reference(sys_, grid, nfreq)
+        except Exception as e:   # the checker is a convenience here: fall back, and say why
+            _log(f"reference cpu baseline failed ({type(e).__name__}: {e}); timing the port instead")
+            out = cpu_baseline_port(sys_, grid, nfreq)
+            out["reference_failed"] = f"{type(e).__name__}: {e}"[:300]
+            return out
+    return cpu_baseline_port(sys_, grid, nfreq)
+
+
+def cpu_baseline_port(sys_, grid, nfreq):
     """FP64 oracle timed on a bounded sample of the workload and scaled to s/freq (an
     extrapolation: a full oracle run of this workload takes hours)."""
     from oracle import rpa

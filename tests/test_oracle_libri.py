@@ -15,74 +15,17 @@ This pins the oracle's label sums, cell folding (LibRI's ``cal_period``) and R s
 k-point systems against the reference's code, not against a restatement of it. Planted cell-index
 mistakes in the oracle must fail the comparisons.
 """
-import os
-import struct
-import subprocess
-
 import numpy as np
 import pytest
 
 from librpa_amd import synth
 from oracle import gw as ogw
+from oracle import ref_io
 from oracle import rpa
 
-REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                   "oracle", "_ref", "ref_libri")
-
 pytestmark = pytest.mark.skipif(
-    not os.access(REF, os.X_OK),
+    not ref_io.available(),
     reason="oracle/_ref/ref_libri is not built (make -C oracle needs the reference tree)")
-
-
-def _pack_blocks(blocks):
-    out = [struct.pack("<i", len(blocks))]
-    for (I, J, R, data) in blocks:
-        out.append(struct.pack("<5i", I, J, *R))
-        out.append(np.ascontiguousarray(data, dtype="<f8").tobytes())
-    return b"".join(out)
-
-
-def _dense_blocks(s, basis, M):
-    """Blocks (I, J, R, M[R][I, J]) of a dense per-cell matrix for every atom pair and BvK cell."""
-    return [(I, J, R, M[R][basis.sl(I), basis.sl(J)])
-            for I in range(s.natom) for J in range(s.natom) for R in rpa.r_grid(s.period)]
-
-
-def _write_input(path, s, second, third):
-    parts = [struct.pack("<4i", s.natom, *s.period),
-             np.asarray(s.latvec, dtype="<f8").tobytes(),
-             np.asarray(s.positions, dtype="<f8").tobytes(),
-             np.asarray(s.nao, dtype="<i4").tobytes(),
-             np.asarray(s.nabf, dtype="<i4").tobytes()]
-    cs = []
-    for (I, J, R, blk) in s.cs:
-        ni, nj, nmu = int(s.nao[I]), int(s.nao[J]), int(s.nabf[I])
-        cs.append((I, J, tuple(int(x) for x in R), blk.reshape(ni, nj, nmu).transpose(2, 0, 1)))
-    parts += [_pack_blocks(cs), _pack_blocks(second), _pack_blocks(third)]
-    with open(path, "wb") as f:
-        f.write(b"".join(parts))
-
-
-def _read_output(path):
-    raw = open(path, "rb").read()
-    (count,), off = struct.unpack_from("<i", raw, 0), 4
-    out = {}
-    for _ in range(count):
-        I, J, r0, r1, r2, nr, nc = struct.unpack_from("<7i", raw, off)
-        off += 28
-        out[(I, J, (r0, r1, r2))] = np.frombuffer(raw, "<f8", nr * nc, off).reshape(nr, nc).copy()
-        off += 8 * nr * nc
-    assert off == len(raw)
-    return out
-
-
-def _run_libri(tmp_path, mode, s, second, third):
-    fin, fout = str(tmp_path / f"{mode}_in.bin"), str(tmp_path / f"{mode}_out.bin")
-    _write_input(fin, s, second, third)
-    env = dict(os.environ, OMP_NUM_THREADS="1")
-    subprocess.run([REF, mode, fin, fout], check=True, timeout=120, env=env,
-                   stdout=subprocess.PIPE, stderr=subprocess.PIPE)
-    return _read_output(fout)
 
 
 CASES = [
@@ -101,8 +44,7 @@ def _inputs(kw, tau=0.37):
 
 
 def _libri_chi0s(tmp_path, s, Gp, Gn):
-    ao = rpa.Basis(s.nao)
-    return _run_libri(tmp_path, "chi0s", s, _dense_blocks(s, ao, Gp), _dense_blocks(s, ao, Gn))
+    return ref_io.chi0s(s, Gp, Gn, timeout=120)[0]
 
 
 def _w_of_r(s, seed=0):
@@ -114,8 +56,7 @@ def _w_of_r(s, seed=0):
 
 
 def _libri_sigmas(tmp_path, s, G, W):
-    return _run_libri(tmp_path, "sigmas", s, _dense_blocks(s, rpa.Basis(s.nabf), W),
-                      _dense_blocks(s, rpa.Basis(s.nao), G))
+    return ref_io.sigmas(s, G, W, timeout=120)
 
 
 def _compare(ref, mine_of_key, Rs, what):
@@ -185,3 +126,19 @@ def test_sigma_comparison_detects_planted_cell_mistake(tmp_path):
     scale = max(np.abs(v).max() for v in ref.values())
     err = max(np.abs(wrong[k[2]][ao.sl(k[0]), ao.sl(k[1])] - v).max() for k, v in ref.items())
     assert err > 1e-6 * scale
+
+
+def test_bench_reference_cpu_baseline_runs():
+    # the bench's CPU baseline leg drives the same checker on atom samples of its workload
+    import bench
+    from oracle import tfgrids
+    s = synth.tiny(natom=8, period=(1, 1, 1), nao=(3, 2), nabf=(4, 3), nocc=(1, 1), a=9.0, seed=2)
+    emin, emax = tfgrids.e_min_max(s.eps, s.occ, s.nspin, s.nk)
+    grid = tfgrids.minimax(s.nfreq, emin, emax)
+    r = bench.cpu_baseline_reference(s, grid, s.nfreq)
+    assert r["kind"] == "reference" and r["value"] > 0
+    assert set(r["parts"]["libri_samples"]) == {"2", "4"}
+    # samples keep the mix of atom kinds (n_ao, n_abf): here 4 + 4 atoms of two kinds
+    pick = bench._balanced_atoms(s, 4)
+    assert len(pick) == 4 and len(set(pick)) == 4
+    assert sorted(int(s.nao[i]) for i in pick) == [2, 2, 3, 3]
