@@ -49,6 +49,15 @@ int librpa_amd_get_chi0q_block(LibrpaHandler *h, int ifreq, int iq, int I, int J
 int librpa_amd_get_tfgrids(LibrpaHandler *h, const LibrpaOptions *p_opts, double *tau, double *tau_w,
                            double *omega, double *omega_w, double *cos_t2f, double *cos_f2t, double *sin_t2f);
 
+/* The macroscopic dielectric function eps_M(i omega) given at n_in imaginary frequencies, on the
+ * n_out frequencies omega_out, as the head replacement of G0W0 uses it (option_dielect_func):
+ * 0 as given (n_in == n_out), 1 LibRPA's cubic spline, 2 its double Havriliak-Negami fit.
+ * LibRPA keeps this step internal (interpolate_dielec_func, src/core/dielecmodel.cpp:65-104);
+ * it is exported here so that callers can see the values W is built with.  Host only.
+ * Returns n_out, or -1 on error. */
+int librpa_amd_interpolate_dielec_func(int option, int n_in, const double *omega_in, const double *df_in,
+                                       int n_out, const double *omega_out, double *df_out);
+
 /* Device timing table as JSON {"region": [ms, calls, flops, bytes], ...}.
  * Returns the length written (excluding the terminator). */
 int librpa_amd_profile_json(char *buf, int len);

@@ -152,6 +152,7 @@ _SIGS = {
     "librpa_amd_get_chi0s_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "librpa_amd_get_chi0q_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "librpa_amd_get_tfgrids": (C.c_int, [_hp, _op, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "librpa_amd_interpolate_dielec_func": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]),
     "librpa_amd_get_wc_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "librpa_amd_get_sigma_block": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _dp]),
     "librpa_amd_get_sigc_ks": (C.c_int, [_hp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
@@ -210,6 +211,20 @@ def _d(a):
 def _f(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a, a.ctypes.data_as(_fp)
+
+
+def interpolate_dielec_func(option, omega_in, df_in, omega_out):
+    """eps_M(i omega) given at omega_in, on omega_out (option_dielect_func 0, 1 or 2): the values
+    the G0W0 head replacement uses, from the library's host code."""
+    om, po = _d(omega_in)
+    df, pd = _d(df_in)
+    tg, pt = _d(omega_out)
+    out = np.zeros(len(tg))
+    n = load().librpa_amd_interpolate_dielec_func(int(option), len(om), po, pd, len(tg), pt, out.ctypes.data_as(_dp))
+    _check()
+    if n != len(tg):
+        raise LibrpaError("librpa_amd_interpolate_dielec_func failed")
+    return out
 
 
 def screen_train(X, t, iters=2000, lr=3e-3, seed=0):

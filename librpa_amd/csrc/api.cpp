@@ -12,6 +12,7 @@
 
 #include "../../include/librpa.h"
 #include "dataset.h"
+#include "dielec.h"
 #include "gw.h"
 #include "rpa.h"
 
@@ -785,6 +786,20 @@ int librpa_amd_get_tfgrids(LibrpaHandler *h, const LibrpaOptions *p_opts, double
             if (cos_f2t) std::memcpy(cos_f2t, g.cos_f2t.data(), n * n * 8);
             if (sin_t2f) std::memcpy(sin_t2f, g.sin_t2f.data(), n * n * 8);
             return g.n;
+        },
+        -1);
+}
+
+int librpa_amd_interpolate_dielec_func(int option, int n_in, const double *omega_in, const double *df_in,
+                                       int n_out, const double *omega_out, double *df_out) {
+    return guarded(
+        [&]() -> int {
+            if (n_in < 0 || n_out < 0) LRA_THROW("dielectric function: negative size");
+            const std::vector<double> om(omega_in, omega_in + n_in), df(df_in, df_in + n_in);
+            const std::vector<double> tgt(omega_out, omega_out + n_out);
+            const auto v = lra::interpolate_dielec(option, om, df, tgt);
+            std::memcpy(df_out, v.data(), v.size() * sizeof(double));
+            return n_out;
         },
         -1);
 }

@@ -41,6 +41,7 @@
 #include <map>
 #include <set>
 
+#include "dielec.h"
 #include "gw_kernels.h"
 
 namespace lra {
@@ -1123,18 +1124,12 @@ void gw_build_sigma(Dataset &d, const LibrpaOptions &o) {
     if (o.sqrt_coulomb_threshold < 0) LRA_THROW("sqrt_coulomb_threshold must not be negative");
     lra_prepare_grid(d, o);
     // head of the dielectric matrix from the macroscopic dielectric function
-    // (librpa_get_g0w0_sigc_kgrid path, src/api/compute_g0w0.cpp:102-118): option 0 takes the
-    // input values as they are, one per grid frequency (interpolate_dielec_func,
-    // src/core/dielecmodel.cpp:65-104)
+    // (librpa_get_g0w0_sigc_kgrid path, src/api/compute_g0w0.cpp:102-118), on the grid's
+    // frequencies as interpolate_dielec_func (src/core/dielecmodel.cpp:65-104) puts it:
+    // 0 as given, 1 cubic spline, 2 Havriliak-Negami fit
     std::vector<double> head;
-    if (o.replace_w_head == LIBRPA_SWITCH_ON && !d.dielect_omega.empty()) {
-        if (o.option_dielect_func != 0)
-            LRA_THROW("option_dielect_func " + std::to_string(o.option_dielect_func) +
-                      ": only 0 (the dielectric function given on the frequency grid) is in this build");
-        if ((int)d.dielect_val.size() != d.grid.n)
-            LRA_THROW("option_dielect_func 0 needs the dielectric function at the nfreq grid frequencies");
-        head = d.dielect_val;
-    }
+    if (o.replace_w_head == LIBRPA_SWITCH_ON && !d.dielect_omega.empty())
+        head = interpolate_dielec(o.option_dielect_func, d.dielect_omega, d.dielect_val, d.grid.omega);
     auto eng = d.rpa;
     if (!eng || eng->prec != d.chi0_prec || eng->mixed != d.chi0_mixed || eng->planned_ntau != d.grid.n) {
         d.rpa.reset();
