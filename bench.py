@@ -279,23 +279,11 @@ def cpu_baseline_port(sys_, grid, nfreq):
     t_w = max(t_one - t_out, 0.0)
     n_out = sys_.natom * (sys_.natom + 1) // 2 * len(Rs)
     chi0_per_tau = t_g + sys_.natom * t_w + n_out * t_out
-    # Ec: one omega at the full matrix size (the n^3 scaling below is then 1)
-    n = int(sys_.nabf.sum())
-    m = n
-    rng = np.random.default_rng(0)
-    b = rng.standard_normal((m, m))
-    V = b @ b.T / m + 1e-3 * np.eye(m)
-    x = rng.standard_normal((m, m))
-    chi = -(x @ x.T) / m
-    t0 = time.perf_counter()
-    Pi = V @ chi
-    rpa.ln_det_ref(np.eye(m) - Pi)
-    t_ec1 = (time.perf_counter() - t0) * (n / m) ** 3
-    total = chi0_per_tau * len(grid["tau"]) + t_ec1 * nfreq * sys_.nk
+    total = chi0_per_tau * len(grid["tau"]) + _ec_port_seconds(sys_, nfreq)
     return dict(value=round(total / nfreq, 3), unit="s/freq", cores=threads, kind="port", extrapolated=True,
                 sample=(f"numpy FP64 oracle: G(+-tau) and W(J={J}) plus {nI} output blocks at tau {it}, "
-                        f"scaled to {n_out} blocks x {len(grid['tau'])} tau; Ec: one omega at n={m} scaled "
-                        f"by (n_abf/{m})^3; total / nfreq"))
+                        f"scaled to {n_out} blocks x {len(grid['tau'])} tau; Ec: one ln det per omega and q "
+                        f"at n_abf; total / nfreq"))
 
 
 def copy_calib(nbytes):
