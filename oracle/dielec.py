@@ -25,7 +25,8 @@ Pins: tests/test_oracle_ref_dielec.py holds both against the reference's own int
 and fitting.cpp (oracle/ref_dielec.cpp); the spline passes through the data with continuous value
 and second derivative (its first derivative jumps at the knots, so unlike a textbook spline it does
 not reproduce a straight line), and the fit reproduces data that two equal terms represent exactly
-(1 + A / (B + u^2)).
+(1 + A / (B + u^2)). tests/test_oracle_known_answers.py runs the reference's own known answer for
+the fit (src/test/test_fitting.cpp:9-40: sin(a x) from a = 1 recovers a to 1e-8).
 """
 from __future__ import annotations
 
