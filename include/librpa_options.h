@@ -40,6 +40,9 @@ typedef struct {
     LibrpaSwitch use_scalapack_gw_wc;
     LibrpaSwitch use_cholesky_gw_wc;
     LibrpaSwitch replace_w_head;
+    /* how the dielectric function of librpa_set_dielect_func_imagfreq reaches the grid
+     * frequencies (src/core/dielecmodel.cpp:65-104): 0 as given, 1 cubic spline,
+     * 2 double Havriliak-Negami fit; 3 (head and wings) is not in this library */
     int option_dielect_func;
     LibrpaSwitch use_2d_dielectric;
     LibrpaSwitch load_sigc_from_file;
