@@ -19,7 +19,9 @@ ranges, and for the four regression cases' own ranges:
   1e-2: large n on a narrow energy range) the weights reach 1e8 and beyond in greenX too, both
   residuals are set by rounding, and they differ by up to 3.4x; the bound there is 4x.
 
-A planted mistake in the fit's x-node count fails the matrix comparison.
+A planted mistake in the fit's x-node count fails the matrix comparison. With the Tikhonov
+regularisation LibRPA passes on (minimax_regulation), the fits that are singular without it agree
+entry by entry (1e-8 at 1e-6, 1e-5 at 1e-8).
 """
 import json
 import os
@@ -53,12 +55,13 @@ CASES = [(n, a, b) for n in SIZES for (a, b) in RANGES] + \
         [(GOLD[k]["n"],) + tuple(_gold_range(GOLD[k])) for k in sorted(GOLD)]
 
 
-def _run_greenx(cases, tmp):
+def _run_greenx(cases, tmp, reg=None):
     fin, fout = os.path.join(tmp, "mm_in.bin"), os.path.join(tmp, "mm_out.bin")
+    regs = reg if reg is not None else [0.0] * len(cases)
     with open(fin, "wb") as f:
         f.write(struct.pack("<i", len(cases)))
-        for (n, a, b) in cases:
-            f.write(struct.pack("<iddd", n, a, b, 0.0))
+        for (n, a, b), r in zip(cases, regs):
+            f.write(struct.pack("<iddd", n, a, b, r))
     env = dict(os.environ, MKL_NUM_THREADS="2", OMP_NUM_THREADS="2")
     subprocess.run([REF, fin, fout], check=True, timeout=300, env=env,
                    stdout=subprocess.PIPE, stderr=subprocess.PIPE)
@@ -184,7 +187,7 @@ def test_matrix_comparison_detects_planted_x_nodes(greenx, monkeypatch):
         _check(tfgrids.minimax(n, a, b), greenx[(n, a, b)], a, b, (n, a, b))
 
 
-def _library_grid(n, emin, emax):
+def _library_grid(n, emin, emax, reg=0.0):
     import librpa_amd as L
     # one k, six bands: gap emin across the Fermi level, band width emax
     eps = np.array([-0.6, -emin / 2 - 0.05, -emin / 2, emin / 2, 1.0, -0.6 + emax])
@@ -193,7 +196,7 @@ def _library_grid(n, emin, emax):
     h = L.Handler()
     h.set_scf_dimension(1, 1, 6, 6)
     h.set_wg_ekb_efermi(1, 1, 6, occ, eps, 0.0)
-    return h.get_tfgrids(L.Options(nfreq=n, tfgrids_type=L.MINIMAX))
+    return h.get_tfgrids(L.Options(nfreq=n, tfgrids_type=L.MINIMAX, minimax_regulation=reg))
 
 
 LIBRARY_RANGES = RANGES[:1] + RANGES[2:3]   # a wide and a narrow range: every size, well posed and singular
@@ -219,3 +222,28 @@ def test_library_grids_equal_greenx(greenx, lib):
         g = h.get_tfgrids(L.Options(nfreq=r["n"], tfgrids_type=L.MINIMAX))
         a, b = _gold_range(r)
         _check(g, greenx[(r["n"], a, b)], a, b, ("library", key))
+
+
+# Tikhonov-regularised fits (minimax_regulation, passed to greenX by LibRPA's
+# src/api/dataset_helper.cpp:20-28): the filter s / (reg^2 + s^2) bounds the weights, so the
+# fits that are singular without it agree entry by entry; 1e-12 and the 32-point H2O range are
+# the reference's own unit test (src/test/test_timefreq.cpp:124-137), still singular there.
+EMIN_H2O = 0.302066195818780992e-2 + 0.260399583086844855e0
+EMAX_H2O = 0.322960860707412234e1 + 0.187886173733204913e2
+REG_CASES = [(32, EMIN_H2O, EMAX_H2O, 1e-6), (34, 0.25, 3.6, 1e-6), (24, 0.35, 60.0, 1e-6),
+             (32, EMIN_H2O, EMAX_H2O, 1e-8), (16, 0.5, 8.0, 1e-8), (32, EMIN_H2O, EMAX_H2O, 1e-12)]
+REG_TOL = {1e-6: 1e-8, 1e-8: 1e-5}
+
+
+def test_regularised_fits_equal_greenx(tmp_path, lib):
+    refs = _run_greenx([c[:3] for c in REG_CASES], str(tmp_path), reg=[c[3] for c in REG_CASES])
+    for (n, a, b, reg), ref in zip(REG_CASES, refs):
+        for who, mine in (("oracle", tfgrids.minimax(n, a, b, reg)), ("library", _library_grid(n, a, b, reg))):
+            for k in ("tau", "tau_w", "omega", "omega_w"):
+                assert np.abs(mine[k] - ref[k]).max() <= 1e-14 * np.abs(ref[k]).max(), (who, n, reg, k)
+            res_mine, res_ref = _residuals(mine, a, b), _residuals(ref, a, b)
+            for k in KINDS:
+                assert res_mine[k] <= 4.0 * res_ref[k] + 1e-14, (who, n, reg, k, res_mine[k], res_ref[k])
+                if reg in REG_TOL:
+                    d = np.abs(mine[k] - ref[k]).max() / np.abs(ref[k]).max()
+                    assert d <= REG_TOL[reg], (who, n, reg, k, d)
